@@ -1,0 +1,114 @@
+"""Seeded synthetic inputs for the Stokes hot path.
+
+The reference drives the solver with temperature-dependent nodal viscosity
+and density (``rock_prop.m``) and GPlates surface velocities
+(``SETUP_TEST/bc_sph.m:329-423``); neither data source is available offline,
+so these are synthetic stand-ins of the same shape:
+
+* temperature: linear conductive profile plus a seeded random real spherical
+  harmonic perturbation (degree <= 8, max amplitude ``amp``) with a radial
+  sine envelope, optional Gaussian plume at the CMB;
+* viscosity ``exp(-ln(contrast) * T)`` and density ``1 - (T - T_ref)`` at every
+  node (only vertex values enter the assembly, calc_element_visc.m:5-7);
+* boundary conditions in the rotated (theta, phi, r) dof frame
+  (bc_sph.m:381-423): free slip at the CMB (u_r = 0), prescribed plate
+  velocities (u_theta, u_phi) and u_r = 0 at the surface, from ``nplate``
+  seeded spherical-Voronoi plates with random Euler poles of unit rate.
+"""
+import numpy as np
+from scipy.special import lpmv
+
+
+def rotation_frames(X):
+    """Per-node unit vectors (theta_hat, phi_hat, r_hat) as the columns of
+    the 3x3 block of RR (make_rotation_matrix.m:69-110).  Returns (n,3,3)."""
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    th = np.arctan2(np.sqrt(x ** 2 + y ** 2), z)
+    ph = np.arctan2(y, x)
+    ph = np.where(ph < 0, ph + 2 * np.pi, ph)
+    ct, st, cp, sp_ = np.cos(th), np.sin(th), np.cos(ph), np.sin(ph)
+    R = np.zeros((len(X), 3, 3))
+    R[:, 0, 0] = cp * ct
+    R[:, 1, 1] = cp
+    R[:, 2, 2] = ct
+    R[:, 0, 1] = -sp_
+    R[:, 0, 2] = cp * st
+    R[:, 1, 0] = sp_ * ct
+    R[:, 1, 2] = sp_ * st
+    R[:, 2, 0] = -st
+    R[:, 2, 1] = 0.0
+    return R
+
+
+def spherical_harmonic_field(th, ph, lmax, rng):
+    """Random real combination of spherical harmonics of degree 1..lmax."""
+    f = np.zeros_like(th)
+    ct = np.cos(th)
+    for l in range(1, lmax + 1):
+        for m in range(0, l + 1):
+            P = lpmv(m, l, ct)
+            a, b = rng.standard_normal(2) / (l + 1)
+            f += a * P * np.cos(m * ph)
+            if m > 0:
+                f += b * P * np.sin(m * ph)
+    return f
+
+
+def temperature(mesh, seed=0, amp=0.05, lmax=8, plume=None):
+    S = mesh.GCOORD_SPH
+    r = S[:, 2]
+    ri, ro = mesh.r_cmb, mesh.r_surf
+    T = (ro - r) / (ro - ri)
+    rng = np.random.default_rng(seed)
+    f = spherical_harmonic_field(S[:, 0], S[:, 1], lmax, rng)
+    f *= amp / np.abs(f).max()
+    T = T + f * np.sin(np.pi * (r - ri) / (ro - ri))
+    if plume is not None:
+        dT, rad, th0, ph0 = plume
+        c = np.array([np.sin(th0) * np.cos(ph0), np.sin(th0) * np.sin(ph0), np.cos(th0)]) * ri
+        d = np.linalg.norm(mesh.GCOORD - c[None, :], axis=1)
+        T = T + dT * np.exp(-(d / rad) ** 2)
+    return T
+
+
+def material(T, visc_contrast, T_ref=0.5):
+    Visc = np.exp(-np.log(visc_contrast) * T)
+    Dens = 1.0 - (T - T_ref)
+    return Visc, Dens
+
+
+def physics(Ra=1e5):
+    """g * Bscale * (Dens - Dens0) = Ra * (T - T_ref), radially outward."""
+    return dict(g=-1.0, Bscale=Ra, Dens0=1.0, minVisc=None, maxVisc=None)
+
+
+def plate_bcs(mesh, seed=1, nplate=6):
+    """UBC.ifix / UBC.vfix (0-based dofs, rotated frame) following the dof
+    choice of bc_sph.m:381-423."""
+    rng = np.random.default_rng(seed)
+    centers = rng.standard_normal((nplate, 3))
+    centers /= np.linalg.norm(centers, axis=1)[:, None]
+    poles = rng.standard_normal((nplate, 3))
+    poles /= np.linalg.norm(poles, axis=1)[:, None]
+    pid = mesh.PointID
+    X = mesh.GCOORD
+    frames = rotation_frames(X)
+    ifix, vfix = [], []
+    for node in np.nonzero(pid > 0)[0]:
+        if pid[node] == 301:
+            ifix.append(3 * node + 2)
+            vfix.append(0.0)
+        elif pid[node] == 306:
+            x = X[node]
+            p = int(np.argmax(centers @ (x / np.linalg.norm(x))))
+            v = np.cross(poles[p], x)
+            ifix += [3 * node, 3 * node + 1, 3 * node + 2]
+            vfix += [float(v @ frames[node][:, 0]), float(v @ frames[node][:, 1]), 0.0]
+    return np.array(ifix, dtype=np.int64), np.array(vfix)
+
+
+def make_inputs(mesh, visc_contrast=1e4, Ra=1e5, seed_T=0, seed_plates=1, plume=None):
+    T = temperature(mesh, seed=seed_T, plume=plume)
+    Visc, Dens = material(T, visc_contrast)
+    ifix, vfix = plate_bcs(mesh, seed=seed_plates)
+    return dict(T=T, Visc=Visc, Dens=Dens, phys=physics(Ra), ifix=ifix, vfix=vfix)

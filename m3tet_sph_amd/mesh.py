@@ -1,0 +1,476 @@
+"""Spherical-shell P2 tetrahedral meshes and their nested multigrid hierarchy.
+
+This is the set-up layer that produces the mesh arrays the Stokes hot path
+consumes (the reference loads a coarse mesh from a .mat file and refines it;
+here the coarse mesh is an icosahedral shell built from scratch).  Every rule
+that decides a coordinate or a numbering follows the reference:
+
+* P1 -> P2 promotion with curved edges and the four element classes:
+  ``mfiles_SPH/tetmesh_p1_to_p2_sph.m:183-232`` (classes via
+  ``check_els_in_cone.m:59-76`` and ``check_phi.m:59-71``), mid-side nodes in
+  the original spherical frame (``:574-634``, phi unwrapped by
+  ``check_ang_dist_phi`` ``:258-283``), in the 90-degree X-rotated frame for
+  edges that touch the cone (``:379-508``) and in the 180-degree Z-rotated
+  frame for elements crossing phi=2pi (``:287-375``).  Edge numbering is the
+  first-appearance order of ``find_unique_bars`` (``:238-254``).
+* Nested 8-way refinement: ``mfiles_M3TET/tetmesh_p2_to_p1.m:27-116``
+  (shortest-diagonal split, children 8(e-1)+1..8e) inside
+  ``mfiles_SPH/tetmesh_refine_sph.m:56-89``.
+* Cubic (20-node) geometry of the elements that cross the cone boundary:
+  ``mfiles_SPH/cubic_els_in_cone_iso.m:1-346``.
+* Inter-mesh transfer operator: ``mfiles_M3TET/intermesh_transfer_matrix.m``
+  evaluated on the straight (affine) coarse geometry
+  (``create_multigrid_meshes_3d_sph.m:54-64,79-82``).  On that geometry the
+  local coordinate of every fine node inside its parent is an exact multiple
+  of 1/4, so the weights follow from the refinement tables alone; they are
+  computed here from those tables (the reference computes the same numbers by
+  inverting the straight element map and rounding to 1e-3).
+
+Arrays are 0-based here; the C ABI receives the reference's 1-based layout.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+PI = np.pi
+THETA_CONE_DEG = 45.0
+ROT_X90_CCW = np.array([[1.0, 0, 0], [0, 0, -1.0], [0, 1.0, 0]])
+ROT_X90_CW = np.array([[1.0, 0, 0], [0, 0, 1.0], [0, -1.0, 0]])
+ROT_Z180 = np.array([[-1.0, 0, 0], [0, -1.0, 0], [0, 0, 1.0]])
+
+# local edges of a tetrahedron in the reference's P2 node order 5..10
+# (tetmesh_p1_to_p2_sph.m:574-580)
+TET_EDGES = np.array([[0, 1], [1, 2], [2, 3], [3, 0], [0, 2], [3, 1]])
+# children of a P2 tetrahedron for the three diagonal choices
+# (tetmesh_p2_to_p1.m:46-73), 0-based local P2 node indices
+EL2CHILD = np.array([
+    [[0, 4, 8, 7], [4, 1, 5, 9], [8, 5, 2, 6], [7, 9, 6, 3],
+     [9, 7, 6, 4], [5, 8, 4, 6], [7, 6, 4, 8], [9, 4, 6, 5]],
+    [[0, 4, 8, 7], [4, 1, 5, 9], [8, 5, 2, 6], [7, 9, 6, 3],
+     [4, 8, 7, 5], [9, 6, 5, 7], [8, 7, 5, 6], [4, 5, 7, 9]],
+    [[0, 4, 8, 7], [4, 1, 5, 9], [8, 5, 2, 6], [7, 9, 6, 3],
+     [7, 4, 8, 9], [6, 5, 9, 8], [4, 8, 9, 5], [7, 9, 8, 6]],
+])
+DIAG2NOD = np.array([[4, 6], [5, 7], [8, 9]])
+
+
+def cart2sph(X):
+    """(x,y,z) -> (theta, phi, r), phi in [0, 2pi) (cartesian2spherical.m)."""
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    r = np.sqrt(x ** 2 + y ** 2 + z ** 2)
+    th = np.arctan2(np.sqrt(x ** 2 + y ** 2), z)
+    ph = np.arctan2(y, x)
+    ph = np.where(ph < 0, ph + 2 * PI, ph)
+    return np.stack([th, ph, r], axis=1)
+
+
+def sph2cart(S):
+    """(theta, phi, r) -> (x,y,z) (spherical2cartesian.m)."""
+    th, ph, r = S[:, 0], S[:, 1], S[:, 2]
+    return np.stack([r * np.sin(th) * np.cos(ph),
+                     r * np.sin(th) * np.sin(ph),
+                     r * np.cos(th)], axis=1)
+
+
+def unique_first_order(pairs):
+    """Unique unordered rows in order of first appearance.
+
+    Mirrors ``find_unique_bars`` / ``unique_keep_order``: returns the unique
+    rows (orientation of their first occurrence) and, for every input row, the
+    index of its unique row.
+    """
+    s = np.sort(pairs, axis=1).astype(np.int64)
+    base = int(s.max()) + 1 if s.size else 1
+    if s.shape[1] == 2:
+        key = s[:, 0] * base + s[:, 1]
+    else:
+        s = np.ascontiguousarray(s)
+        key = s.view(np.dtype((np.void, s.dtype.itemsize * s.shape[1]))).ravel()
+    _, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return pairs[first[order]], rank[inv.ravel()]
+
+
+def icosphere(level):
+    """Unit icosahedral sphere subdivided ``level`` times (10*4^L+2 vertices)."""
+    p = (1.0 + 5.0 ** 0.5) / 2.0
+    V = np.array([(-1, p, 0), (1, p, 0), (-1, -p, 0), (1, -p, 0),
+                  (0, -1, p), (0, 1, p), (0, -1, -p), (0, 1, -p),
+                  (p, 0, -1), (p, 0, 1), (-p, 0, -1), (-p, 0, 1)], dtype=float)
+    V /= np.linalg.norm(V, axis=1)[:, None]
+    F = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+                  (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+                  (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+                  (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)])
+    for _ in range(level):
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+        uniq, idx = unique_first_order(e)
+        mid = V[uniq[:, 0]] + V[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1)[:, None]
+        nv = len(V)
+        V = np.concatenate([V, mid])
+        nf = len(F)
+        m01, m12, m20 = (idx[:nf] + nv, idx[nf:2 * nf] + nv, idx[2 * nf:] + nv)
+        a, b, c = F[:, 0], F[:, 1], F[:, 2]
+        F = np.concatenate([np.stack([a, m01, m20], 1), np.stack([m01, b, m12], 1),
+                            np.stack([m20, m12, c], 1), np.stack([m01, m12, m20], 1)])
+    return V, F
+
+
+def tet_volume6(X, EL):
+    """6 x signed volume with the reference's sign convention
+    det[x1-x4, x2-x4, x3-x4] (calc_tetra_volume.m, local node 4 at the
+    origin of the master element)."""
+    a = X[EL[:, 0]] - X[EL[:, 3]]
+    b = X[EL[:, 1]] - X[EL[:, 3]]
+    c = X[EL[:, 2]] - X[EL[:, 3]]
+    return np.einsum("ij,ij->i", a, np.cross(b, c))
+
+
+def shell_p1(level, nlayer, r_in, r_out):
+    """Icosahedral shell: each prism split into 3 tets (conforming split).
+
+    Vertex id = layer*nsurf + surface id (layer 0 = CMB).  Each prism's three
+    quadrilateral faces are cut along the diagonal from the lower-id bottom
+    vertex to the higher-id top vertex, which makes neighbouring prisms agree.
+    Returns (GCOORD (n,3), EL2NOD (nel,4), PointID (n,)) with positive volumes.
+    """
+    V, F = icosphere(level)
+    ns = len(V)
+    radii = r_in + (r_out - r_in) * np.arange(nlayer + 1) / nlayer
+    radii[0], radii[-1] = r_in, r_out
+    X = (radii[:, None, None] * V[None, :, :]).reshape(-1, 3)
+    pid = np.zeros(len(X), dtype=np.int32)
+    pid[:ns] = 301
+    pid[nlayer * ns:] = 306
+    Fs = np.sort(F, axis=1)
+    tets = []
+    for k in range(nlayer):
+        A, B, C = (Fs[:, 0] + k * ns, Fs[:, 1] + k * ns, Fs[:, 2] + k * ns)
+        A2, B2, C2 = A + ns, B + ns, C + ns
+        t = np.stack([np.stack([A, B, C, C2], 1), np.stack([A, B, B2, C2], 1),
+                      np.stack([A, A2, B2, C2], 1)], axis=1).reshape(-1, 4)
+        tets.append(t)
+    EL = np.concatenate(tets)
+    neg = tet_volume6(X, EL) < 0
+    EL[neg] = EL[neg][:, [0, 1, 3, 2]]
+    return X, EL, pid
+
+
+def _midpoints_rotated(X4, bars, R, Rinv):
+    """Mid-side nodes straight in a rotated spherical frame, returned in the
+    original spherical frame (tetmesh_p1_to_p2_sph.m:329-373, 433-506)."""
+    S = cart2sph(X4 @ R.T)
+    mid = 0.5 * (S[bars[:, 0]] + S[bars[:, 1]])
+    return cart2sph(sph2cart(mid) @ Rinv.T)
+
+
+def classify(SPH4, EL4, theta_cone=THETA_CONE_DEG):
+    """Element classes (check_els_in_cone.m:59-76, check_phi.m:59-71)."""
+    tc = theta_cone * PI / 180.0
+    th = SPH4[:, 0]
+    inc = (th <= tc) | (th >= PI - tc)
+    cnt = inc[EL4].sum(axis=1)
+    ph = SPH4[:, 1][EL4]
+    pairs = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    cross = np.zeros(len(EL4), dtype=bool)
+    for a, b in pairs:
+        cross |= np.abs(ph[:, a] - ph[:, b]) > PI
+    out = cnt == 0
+    return dict(
+        out_cone_no_cross_2pi=np.nonzero(out & ~cross)[0],
+        out_cone_cross_2pi=np.nonzero(out & cross)[0],
+        in_cone_no_iso=np.nonzero(cnt >= 3)[0],
+        in_cone_iso=np.nonzero((cnt > 0) & (cnt < 3))[0],
+    ), inc
+
+
+def p1_to_p2_sph(X4, SPH4, EL4, pid4, r_cmb, r_surf, theta_cone=THETA_CONE_DEG):
+    """P1 -> curved P2 (tetmesh_p1_to_p2_sph.m:183-232, 512-634)."""
+    nel = len(EL4)
+    n4 = len(SPH4)
+    classes, inc = classify(SPH4, EL4, theta_cone)
+    tc = theta_cone * PI / 180.0
+    edges = EL4[:, TET_EDGES].reshape(-1, 2)
+    bars, bar_of_edge = unique_first_order(edges)
+    el2bar = bar_of_edge.reshape(nel, 6)
+    # default: straight in the original spherical frame, phi unwrapped
+    th_e = SPH4[bars, 0]
+    ph_e = SPH4[bars, 1].copy()
+    r_e = SPH4[bars, 2]
+    wrap = np.abs(ph_e[:, 0] - ph_e[:, 1]) > PI
+    ph_e[wrap] = np.where(ph_e[wrap] < PI, ph_e[wrap] + 2 * PI, ph_e[wrap])
+    mids = np.stack([0.5 * th_e.sum(1), 0.5 * ph_e.sum(1), 0.5 * r_e.sum(1)], axis=1)
+    # bars of in-cone elements, except iso-element bars with both ends outside
+    in_cone = np.zeros(nel, dtype=bool)
+    in_cone[classes["in_cone_no_iso"]] = True
+    in_cone[classes["in_cone_iso"]] = True
+    sel = np.zeros(len(bars), dtype=bool)
+    sel[el2bar[in_cone].ravel()] = True
+    iso_bars = np.zeros(len(bars), dtype=bool)
+    iso_bars[el2bar[classes["in_cone_iso"]].ravel()] = True
+    out_both = (~inc[bars[:, 0]]) & (~inc[bars[:, 1]])
+    sel &= ~(iso_bars & out_both)
+    idx = np.nonzero(sel)[0]
+    if len(idx):
+        mids[idx] = _midpoints_rotated(X4, bars[idx], ROT_X90_CCW, ROT_X90_CW)
+    # bars of out-cone elements crossing phi = 2pi: 180-degree Z rotation
+    sel2 = np.zeros(len(bars), dtype=bool)
+    sel2[el2bar[classes["out_cone_cross_2pi"]].ravel()] = True
+    idx2 = np.nonzero(sel2)[0]
+    if len(idx2):
+        mids[idx2] = _midpoints_rotated(X4, bars[idx2], ROT_Z180, ROT_Z180)
+    SPH = np.concatenate([SPH4, mids])
+    X = sph2cart(SPH)
+    pid = np.concatenate([pid4, np.where(pid4[bars[:, 0]] == pid4[bars[:, 1]],
+                                         pid4[bars[:, 0]], 0).astype(np.int32)])
+    SPH[pid == 301, 2] = r_cmb
+    SPH[pid == 306, 2] = r_surf
+    EL10 = np.concatenate([EL4, n4 + el2bar], axis=1).astype(np.int64)
+    return X, SPH, EL10, pid, classes, inc, bars
+
+
+def p2_to_p1(X, EL10):
+    """Split every P2 tet into 8 P1 children along its shortest diagonal."""
+    L = np.stack([np.linalg.norm(X[EL10[:, a]] - X[EL10[:, b]], axis=1)
+                  for a, b in DIAG2NOD], axis=1)
+    isplit = np.argmin(L, axis=1)
+    child = EL10[np.arange(len(EL10))[:, None, None], EL2CHILD[isplit]].reshape(-1, 4)
+    return child, isplit
+
+
+def cubic_iso_coords(X4, SPH4, EL4, els_iso, theta_cone=THETA_CONE_DEG):
+    """Cartesian coordinates of the 20 geometry nodes of each iso element.
+
+    Follows cubic_els_in_cone_iso.m:3-346: edge third-points straight in the
+    original spherical frame for edges with both ends outside the cone
+    (phi unwrapped), straight in the X-rotated frame otherwise; face nodes are
+    the 9-node spherical mean (3-vertex mean in the Z-rotated frame for
+    out-cone faces crossing 2pi, 9-node mean in the X-rotated frame for faces
+    touching the cone).  Node order follows sf_dsf_tet20 (sf_dsf_tet.m:247-287).
+    Returns (n_iso, 20, 3).
+    """
+    tc = theta_cone * PI / 180.0
+    n = len(els_iso)
+    if n == 0:
+        return np.zeros((0, 20, 3))
+    E4 = EL4[els_iso]
+    edges = E4[:, TET_EDGES].reshape(-1, 2)
+    bars, bar_of_edge = unique_first_order(edges)
+    el2bar = bar_of_edge.reshape(n, 6)
+    nb = len(bars)
+    ends = SPH4[bars]                      # (nb, 2, 3)
+    ph = ends[:, :, 1].copy()
+    wrap = np.abs(ph[:, 0] - ph[:, 1]) > PI
+    ph[wrap] = np.where(ph[wrap] < PI, ph[wrap] + 2 * PI, ph[wrap])
+    ends = ends.copy()
+    ends[:, :, 1] = ph
+    third = np.empty((nb, 2, 3))
+    for k, f in enumerate((1.0 / 3.0, 2.0 / 3.0)):
+        third[:, k, :] = ends[:, 0, :] + f * (ends[:, 1, :] - ends[:, 0, :])
+    th_b = SPH4[bars, 0]
+    outc = ((th_b > tc) & (th_b < PI - tc)).sum(axis=1) == 2
+    cc = np.nonzero(~outc)[0]
+    if len(cc):
+        Sr = cart2sph(X4 @ ROT_X90_CCW.T)
+        e0, e1 = Sr[bars[cc, 0]], Sr[bars[cc, 1]]
+        for k, f in enumerate((1.0 / 3.0, 2.0 / 3.0)):
+            t = e0 + f * (e1 - e0)
+            third[cc, k, :] = cart2sph(sph2cart(t) @ ROT_X90_CW.T)
+    # per element: 12 edge nodes oriented along the local edge
+    same = np.all(bars[el2bar] == E4[:, TET_EDGES], axis=2)    # (n, 6)
+    E16_sph = np.empty((n, 16, 3))
+    E16_sph[:, :4] = SPH4[E4]
+    tb = third[el2bar]                                           # (n,6,2,3)
+    first = np.where(same[:, :, None], tb[:, :, 0], tb[:, :, 1])
+    second = np.where(same[:, :, None], tb[:, :, 1], tb[:, :, 0])
+    E16_sph[:, 4:16:2] = first
+    E16_sph[:, 5:16:2] = second
+    X16 = sph2cart(E16_sph.reshape(-1, 3)).reshape(n, 16, 3)
+    # faces: local node lists (cubic_els_in_cone_iso.m:237-240), 0-based
+    faces = np.array([[1, 2, 3, 6, 7, 8, 9, 14, 15],
+                      [0, 2, 3, 12, 13, 8, 9, 10, 11],
+                      [0, 1, 3, 4, 5, 15, 14, 10, 11],
+                      [0, 1, 2, 4, 5, 6, 7, 13, 12]])
+    # global ids for the 16 nodes to identify shared faces
+    gid = np.empty((n, 16), dtype=np.int64)
+    gid[:, :4] = E4
+    big = int(EL4.max()) + 1
+    gid[:, 4:16:2] = big + 2 * np.where(same, el2bar, el2bar) + np.where(same, 0, 1)
+    gid[:, 5:16:2] = big + 2 * el2bar + np.where(same, 1, 0)
+    F9 = gid[:, faces].reshape(-1, 9)
+    F9sph = E16_sph[:, faces].reshape(-1, 9, 3)
+    F9cart = X16[:, faces].reshape(-1, 9, 3)
+    ufaces, face_of = unique_first_order(F9[:, :3])
+    first_row = np.full(len(ufaces), -1, dtype=np.int64)
+    order = np.argsort(face_of, kind="stable")
+    fo_sorted = face_of[order]
+    starts = np.searchsorted(fo_sorted, np.arange(len(ufaces)))
+    first_row = order[starts]
+    fs = F9sph[first_row]                        # (nf, 9, 3)
+    fc = F9cart[first_row]
+    fv = F9[first_row, :3]
+    face_sph = fs.sum(axis=1) / 9.0
+    thv = SPH4[fv, 0]
+    out3 = ((thv > tc) & (thv < PI - tc)).sum(axis=1) == 3
+    phv = SPH4[fv, 1]
+    crossf = ((np.abs(phv[:, 0] - phv[:, 1]) > PI) | (np.abs(phv[:, 1] - phv[:, 2]) > PI)
+              | (np.abs(phv[:, 2] - phv[:, 0]) > PI))
+    sel = np.nonzero(out3 & crossf)[0]
+    if len(sel):
+        Sz = cart2sph(fc[sel, :3].reshape(-1, 3) @ ROT_Z180.T).reshape(-1, 3, 3)
+        c3 = Sz.sum(axis=1) / 3.0
+        face_sph[sel] = cart2sph(sph2cart(c3) @ ROT_Z180.T)
+    sel = np.nonzero(~out3)[0]
+    if len(sel):
+        Sx = cart2sph(fc[sel].reshape(-1, 3) @ ROT_X90_CCW.T).reshape(-1, 9, 3)
+        c9 = Sx.sum(axis=1) / 9.0
+        face_sph[sel] = cart2sph(sph2cart(c9) @ ROT_X90_CW.T)
+    face_x = sph2cart(face_sph)
+    X20 = np.empty((n, 20, 3))
+    X20[:, :4] = X4[E4]
+    X20[:, 4:16] = X16[:, 4:16]
+    X20[:, 16:20] = face_x[face_of.reshape(n, 4)]
+    return X20
+
+
+def _p2_shape(lam):
+    """P2 shape functions at barycentric points lam (..., 4) -> (..., 10)
+    (sf_dsf_tet.m:849-858)."""
+    r, s, t, u = lam[..., 0], lam[..., 1], lam[..., 2], lam[..., 3]
+    return np.stack([r * (2 * r - 1), s * (2 * s - 1), t * (2 * t - 1), u * (2 * u - 1),
+                     4 * r * s, 4 * s * t, 4 * t * u, 4 * r * u, 4 * r * t, 4 * s * u], axis=-1)
+
+
+def child_interp_tables():
+    """Q[isplit, k] (10 fine local nodes x 10 coarse local nodes): P2
+    interpolation weights of the parent at the nodes of child k."""
+    lam10 = np.zeros((10, 4))
+    lam10[:4] = np.eye(4)
+    for j, (a, b) in enumerate(TET_EDGES):
+        lam10[4 + j] = 0.5 * (np.eye(4)[a] + np.eye(4)[b])
+    Q = np.zeros((3, 8, 10, 10))
+    for s in range(3):
+        for k in range(8):
+            v = lam10[EL2CHILD[s, k]]                      # 4 child vertices
+            lamc = np.zeros((10, 4))
+            lamc[:4] = v
+            for j, (a, b) in enumerate(TET_EDGES):
+                lamc[4 + j] = 0.5 * (v[a] + v[b])
+            Q[s, k] = _p2_shape(lamc)
+    return Q
+
+
+@dataclasses.dataclass
+class Level:
+    EL2NOD: np.ndarray          # (nel, 10) int64, 0-based
+    nnod: int
+    nVnod: int
+
+
+@dataclasses.dataclass
+class ShellMesh:
+    """Nested P2 hierarchy; level 0 is the finest (the reference's img=1)."""
+    levels: list
+    GCOORD: np.ndarray          # finest P2 nodes, Cartesian (nnod,3)
+    GCOORD_SPH: np.ndarray      # finest P2 nodes, (theta,phi,r)
+    PointID: np.ndarray         # 301 CMB, 306 surface, 0 interior
+    classes: dict               # finest-level element lists (0-based)
+    X20_iso: np.ndarray         # (n_iso,20,3) cubic geometry of iso elements
+    isplit: list                # isplit[l]: split type of level l+1 elements
+    r_cmb: float
+    r_surf: float
+    theta_cone: float = THETA_CONE_DEG
+
+    @property
+    def nmg(self):
+        return len(self.levels)
+
+    @property
+    def nel(self):
+        return len(self.levels[0].EL2NOD)
+
+    @property
+    def nnod(self):
+        return self.levels[0].nnod
+
+    @property
+    def nVnod(self):
+        return self.levels[0].nVnod
+
+    def class_of_element(self):
+        c = np.zeros(self.nel, dtype=np.int32)
+        for i, k in enumerate(("out_cone_no_cross_2pi", "out_cone_cross_2pi",
+                               "in_cone_no_iso", "in_cone_iso")):
+            c[self.classes[k]] = i
+        return c
+
+    def transfer(self, l):
+        """Node-level interpolation from level l+1 (coarse) to level l (fine):
+        CSR (indptr, indices, weights) with 1 row per fine node.  Row i uses
+        the parent of the last fine element that holds node i, as the
+        reference does (intermesh_transfer_matrix.m:8-23)."""
+        fine = self.levels[l].EL2NOD
+        coarse = self.levels[l + 1].EL2NOD
+        nf = self.levels[l].nnod
+        Q = child_interp_tables()
+        nel_f = len(fine)
+        flat = fine.ravel()
+        order = np.argsort(flat, kind="stable")
+        ends = np.searchsorted(flat[order], np.arange(nf), side="right") - 1
+        lastpos = order[ends]                 # flat index of last occurrence
+        last = lastpos // 10
+        pos = lastpos % 10
+        parent = last // 8
+        k = last % 8
+        W = Q[self.isplit[l][parent], k, pos]            # (nf, 10)
+        cols = coarse[parent]                              # (nf, 10)
+        nz = W != 0.0
+        indptr = np.concatenate([[0], np.cumsum(nz.sum(1))]).astype(np.int64)
+        return indptr, cols[nz].astype(np.int64), W[nz]
+
+    def child_types(self, l):
+        """For every element of level l (l < nmg-1): index isplit*8+k of its
+        10x10 interpolation table from its parent on level l+1."""
+        nel = len(self.levels[l].EL2NOD)
+        par = np.arange(nel) // 8
+        return (self.isplit[l][par] * 8 + np.arange(nel) % 8).astype(np.int32)
+
+
+def build_shell(level, nlayer, nmg, r_in=0.546, r_out=1.0,
+                theta_cone=THETA_CONE_DEG):
+    """Base icosahedral shell (``level`` x ``nlayer``) refined nmg-1 times."""
+    X4, EL4, pid4 = shell_p1(level, nlayer, r_in, r_out)
+    SPH4 = cart2sph(X4)
+    X, SPH, EL10, pid, classes, inc, _ = p1_to_p2_sph(X4, SPH4, EL4, pid4, r_in, r_out,
+                                                      theta_cone)
+    levels = [Level(EL10, len(X), len(X4))]
+    isplits = []
+    for _ in range(nmg - 1):
+        child, isplit = p2_to_p1(X, EL10)
+        isplits.insert(0, isplit)
+        X4, SPH4, EL4, pid4 = X, SPH, child, pid
+        X, SPH, EL10, pid, classes, inc, _ = p1_to_p2_sph(X4, SPH4, EL4, pid4, r_in, r_out,
+                                                          theta_cone)
+        levels.insert(0, Level(EL10, len(X), len(X4)))
+    X20 = cubic_iso_coords(X4, SPH4, EL10[:, :4], classes["in_cone_iso"], theta_cone)
+    return ShellMesh(levels=levels, GCOORD=X, GCOORD_SPH=SPH, PointID=pid, classes=classes,
+                     X20_iso=X20, isplit=isplits, r_cmb=r_in, r_surf=r_out,
+                     theta_cone=theta_cone)
+
+
+# named workloads: base icosahedral level, base layers, multigrid levels
+CONFIGS = {
+    "c0": dict(level=1, nlayer=2, nmg=2),   # finest 3,840 tets, 17,334 U dofs
+    "c1": dict(level=2, nlayer=4, nmg=3),   # finest 245,760 tets, 1,013,958 U dofs
+    "c2": dict(level=2, nlayer=4, nmg=4),   # finest 1,966,080 tets, 7,987,590 U dofs
+    "c4": dict(level=2, nlayer=6, nmg=4),   # finest 2,949,120 tets, 11,919,942 U dofs
+    "tiny": dict(level=1, nlayer=1, nmg=2),  # 240 -> 1,920 tets, unit tests
+}
+
+
+def build_config(name):
+    return build_shell(**CONFIGS[name])

@@ -1,0 +1,13 @@
+"""CPU oracle for the Stokes hot path -- TEST INFRASTRUCTURE ONLY.
+
+A NumPy/SciPy restatement of the reference MATLAB algorithm (JorgeTaramon/
+M3TET_SPH), written to check the HIP implementation.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import it, and only as the checker.  The product (``m3tet_sph_amd``) never
+imports it and fails loudly when its HIP library is missing.
+
+Every function cites the reference file:line it restates.  How each part is
+pinned (known answers, invariants, closed forms) is listed in ``pins.py`` and
+exercised by ``tests/test_oracle_*.py``.  Parts that no reference test or
+mathematical property pins are marked "parity unpinned" in their docstring.
+"""

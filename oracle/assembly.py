@@ -1,0 +1,336 @@
+"""Double-Jacobian P2P1 Stokes assembly, restated (test infrastructure only).
+
+Two independent restatements of the reference:
+
+* ``assemble_opt`` follows the MILAMIN block kernels line by line:
+  mfiles_SPH/assembly_stokes_els_out_cone_no_cross_2pi_opt.m:64-377,
+  ..._out_cone_cross_2pi_opt.m:96-101,385-429 (180-degree Z rotation and its
+  sign table), ..._in_cone_no_iso_opt.m:100-106,390-496 (90-degree X rotation,
+  sign table and dof permutation), ..._in_cone_iso_20_nodes_opt.m:100-142
+  (20-node cubic Jacobian per IP), and the global assembly of
+  assembly_stokes_sph_3d_double_jacobian.m:319-490.
+* ``assemble_std`` follows the per-element 'std' kernels
+  (..._out_cone_no_cross_2pi_std.m:40-271: K = sum B' D B eta w with the
+  '23rd' D matrix of assembly_stokes_sph_3d_double_jacobian.m:195-206), with
+  the un-rotation done as the matrix product R' K R instead of the tables.
+
+Element-packed outputs use the reference layout: K 465 values (LAPACK lower
+packed, column-major over dofs 3(i-1)+c), G 120 values (30x4 column-major),
+M 10 values (lower packed), Fb 30 values.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+from . import elements as E
+
+CLASS_NAMES = ("out_cone_no_cross_2pi", "out_cone_cross_2pi", "in_cone_no_iso", "in_cone_iso")
+
+
+def _vertex_sph(kind, mesh, EL):
+    """Spherical coordinates used for J_SL, per class."""
+    if kind == "out_cone_no_cross_2pi":
+        S = mesh.GCOORD_SPH[EL[:, :4]]
+        return S[..., 0], S[..., 1], S[..., 2]
+    if kind == "out_cone_cross_2pi":
+        S = mesh.GCOORD_SPH[EL[:, :4]]
+        ph = S[..., 1] + np.pi                        # ..._cross_2pi_opt.m:96-101
+        ph = np.where(ph > 2 * np.pi, ph - 2 * np.pi, ph)
+        return S[..., 0], ph, S[..., 2]
+    raise ValueError(kind)
+
+
+def _rot_x90(X):
+    """x'=x, y'=-z, z'=y (..._in_cone_no_iso_opt.m:100-102)."""
+    return X[..., 0], -X[..., 2], X[..., 1]
+
+
+def class_kernel_opt(kind, els, mesh, Visc, Dens, phys, T):
+    """One element class, all its elements at once (the block loop of the
+    reference processes blocks of nelblk elements with identical arithmetic)."""
+    EL = mesh.levels[0].EL2NOD[els]
+    n = len(els)
+    iso = kind == "in_cone_iso"
+    if kind in ("out_cone_no_cross_2pi", "out_cone_cross_2pi"):
+        th, ph, r = _vertex_sph(kind, mesh, EL)
+    elif kind == "in_cone_no_iso":
+        X = mesh.GCOORD[EL[:, :4]]
+        th, ph, r = E.elblk_cartesian2spherical(*_rot_x90(X))
+    else:
+        pos = {int(e): i for i, e in enumerate(mesh.classes["in_cone_iso"])}
+        X = mesh.X20_iso[[pos[int(e)] for e in els]]
+        th, ph, r = E.elblk_cartesian2spherical(*_rot_x90(X))
+    if not iso:
+        Jth, Jph, Jr = th @ T["dN4ds"], ph @ T["dN4ds"], r @ T["dN4ds"]
+        detJ_SL = E.elblk_detA(Jth, Jph, Jr)
+        if np.any(detJ_SL < 0):
+            raise ValueError("negative Jacobi")
+        iJth, iJph, iJr = E.elblk_invert_A(Jth, Jph, Jr, detJ_SL)
+    K = np.zeros((n, 465))
+    G = np.zeros((n, 120))
+    M = np.zeros((n, 10))
+    Fb = np.zeros((n, 30))
+    C1, C2 = 4.0 / 3.0, -2.0 / 3.0
+    vis_v = Visc[EL[:, :4]]
+    den_v = Dens[EL[:, :4]]
+    for ip in range(len(T["IP_w"])):
+        NP = T["NP"][ip]
+        if iso:
+            dN20 = T["dNUds20"][ip]
+            Jth, Jph, Jr = th @ dN20, ph @ dN20, r @ dN20
+            detJ_SL = E.elblk_detA(Jth, Jph, Jr)
+            if np.any(detJ_SL < 0):
+                raise ValueError("negative Jacobi")
+            iJth, iJph, iJr = E.elblk_invert_A(Jth, Jph, Jr, detJ_SL)
+            N20 = T["NU20"][ip]
+            th_ip, ph_ip, r_ip = th @ N20, ph @ N20, r @ N20
+        else:
+            th_ip, ph_ip, r_ip = th @ NP, ph @ NP, r @ NP
+        if np.any(th_ip == 0):
+            raise ValueError("ip has theta = 0")
+        s_th, c_th, s_ph, c_ph = np.sin(th_ip), np.cos(th_ip), np.sin(ph_ip), np.cos(ph_ip)
+        detJ_CS = r_ip ** 2 * s_th
+        J1 = np.stack([r_ip * s_th * c_th * c_ph, -r_ip * s_ph, r_ip ** 2 * s_th ** 2 * c_ph], 1) / detJ_CS[:, None]
+        J2 = np.stack([r_ip * s_th * c_th * s_ph, r_ip * c_ph, r_ip ** 2 * s_th ** 2 * s_ph], 1) / detJ_CS[:, None]
+        J3 = np.stack([-r_ip * s_th ** 2, np.zeros(n), r_ip ** 2 * s_th * c_th], 1) / detJ_CS[:, None]
+        Jx, Jy, Jz = E.elblk_AtxBt(J1, J2, J3, iJth, iJph, iJr)
+        Jx, Jy, Jz = E.elblk_transpose_A(Jx, Jy, Jz)
+        dNds = T["dNUds"][ip]
+        dx, dy, dz = Jx @ dNds.T, Jy @ dNds.T, Jz @ dNds.T
+        weight = detJ_CS * detJ_SL * T["IP_w"][ip]
+        dens = den_v @ NP
+        visc = vis_v @ NP
+        if phys.get("minVisc") is not None:
+            visc = np.maximum(visc, phys["minVisc"])
+        if phys.get("maxVisc") is not None:
+            visc = np.minimum(visc, phys["maxVisc"])
+        if np.any(visc <= 0):
+            raise ValueError("zero or negative viscosity")
+        Fg = phys["g"] * phys["Bscale"] * (dens - phys["Dens0"])
+        w2 = visc * weight
+        idx = 0
+        nn = 10
+        for i in range(nn):
+            for j in range(i, nn):
+                K[:, idx] += w2 * (C1 * dx[:, i] * dx[:, j] + dy[:, i] * dy[:, j] + dz[:, i] * dz[:, j]); idx += 1
+                K[:, idx] += w2 * (C2 * dx[:, i] * dy[:, j] + dy[:, i] * dx[:, j]); idx += 1
+                K[:, idx] += w2 * (C2 * dx[:, i] * dz[:, j] + dz[:, i] * dx[:, j]); idx += 1
+            for j in range(i, nn):
+                if j > i:
+                    K[:, idx] += w2 * (C2 * dy[:, i] * dx[:, j] + dx[:, i] * dy[:, j]); idx += 1
+                K[:, idx] += w2 * (C1 * dy[:, i] * dy[:, j] + dx[:, i] * dx[:, j] + dz[:, i] * dz[:, j]); idx += 1
+                K[:, idx] += w2 * (C2 * dy[:, i] * dz[:, j] + dz[:, i] * dy[:, j]); idx += 1
+            for j in range(i, nn):
+                if j > i:
+                    K[:, idx] += w2 * (C2 * dz[:, i] * dx[:, j] + dx[:, i] * dz[:, j]); idx += 1
+                    K[:, idx] += w2 * (C2 * dz[:, i] * dy[:, j] + dy[:, i] * dz[:, j]); idx += 1
+                K[:, idx] += w2 * (C1 * dz[:, i] * dz[:, j] + dx[:, i] * dx[:, j] + dy[:, i] * dy[:, j]); idx += 1
+        for i in range(4):
+            tmp = (weight * NP[i])[:, None]
+            G[:, i * 30 + 0:i * 30 + 30:3] += tmp * dx
+            G[:, i * 30 + 1:i * 30 + 30:3] += tmp * dy
+            G[:, i * 30 + 2:i * 30 + 30:3] += tmp * dz
+        w3 = weight / visc
+        idx = 0
+        for i in range(4):
+            for j in range(i, 4):
+                M[:, idx] += w3 * NP[i] * NP[j]
+                idx += 1
+        NU = T["NU"][ip]
+        Fb[:, 0::3] += (s_th * c_ph * weight * Fg)[:, None] * NU[None, :]
+        Fb[:, 1::3] += (s_th * s_ph * weight * Fg)[:, None] * NU[None, :]
+        Fb[:, 2::3] += (c_th * weight * Fg)[:, None] * NU[None, :]
+    if kind == "out_cone_cross_2pi":
+        K[:, _sign_rows_z()] *= -1.0                 # ..._cross_2pi_opt.m:408-420
+        G[:, 0::3] *= -1.0
+        G[:, 1::3] *= -1.0
+        Fb[:, 0::3] *= -1.0
+        Fb[:, 1::3] *= -1.0
+    elif kind in ("in_cone_no_iso", "in_cone_iso"):
+        perm, minus = _perm_x90()                    # ..._in_cone_no_iso_opt.m:455-483
+        K = K[:, perm]
+        K[:, minus] *= -1.0
+        Gy = G[:, 1::3].copy()
+        G[:, 1::3] = G[:, 2::3]
+        G[:, 2::3] = -Gy
+        Fy = Fb[:, 1::3].copy()
+        Fb[:, 1::3] = Fb[:, 2::3]
+        Fb[:, 2::3] = -Fy
+    return K, G, M, Fb
+
+
+def _tril_colmajor_index(n=30):
+    """Column-major linear indices (0-based) of tril(n x n)."""
+    idx = np.arange(n * n).reshape(n, n, order="F")
+    rows, cols = np.tril_indices(n)
+    order = np.lexsort((rows, cols))
+    return idx[rows[order], cols[order]]
+
+
+def _sign_rows_z():
+    """..._cross_2pi_opt.m:408-420: packed entries with exactly one z dof."""
+    Ae = np.ones((30, 30))
+    Ae[2::3, :] *= -1
+    Ae[:, 2::3] *= -1
+    return np.nonzero(Ae.ravel(order="F")[_tril_colmajor_index()] == -1)[0]
+
+
+def _perm_x90():
+    """..._in_cone_no_iso_opt.m:455-479 (0-based)."""
+    n = 30
+    minus = _sign_rows_z()
+    perm = np.arange(1, n * n + 1).reshape(n, n, order="F").astype(np.int64)
+    perm = perm - np.ones((n, 1), dtype=np.int64) * np.cumsum(np.arange(n))[None, :]
+    i = np.arange(1, n + 1)[:, None] * np.ones((1, n), dtype=np.int64)
+    j = i.T
+    ind = j > i
+    lin = (i[ind] - 1) * n + (j[ind] - 1)          # sub2ind(j(ind), i(ind)) in F order
+    perm_f = perm.ravel(order="F")
+    perm[ind] = perm_f[(j[ind] - 1) + (i[ind] - 1) * n]
+    _ = lin
+    perm2 = np.arange(1, n + 1).reshape(3, 10, order="F")[[0, 2, 1], :].ravel(order="F") - 1
+    perm = perm[np.ix_(perm2, perm2)]
+    perm = perm.ravel(order="F")[~ind.ravel(order="F")]
+    return perm - 1, minus
+
+
+def element_matrices(mesh, Visc, Dens, phys, kernel=None):
+    """Element-packed KKv (nel,465), GGv (nel,120), MMv (nel,10), Fb (nel,30)."""
+    T = E.sf_tables(14)
+    nel = mesh.nel
+    KKv = np.zeros((nel, 465))
+    GGv = np.zeros((nel, 120))
+    MMv = np.zeros((nel, 10))
+    Fbv = np.zeros((nel, 30))
+    kernel = kernel or class_kernel_opt
+    for kind in CLASS_NAMES:
+        els = mesh.classes[kind]
+        if len(els) == 0:
+            continue
+        K, G, M, F = kernel(kind, els, mesh, Visc, Dens, phys, T)
+        KKv[els], GGv[els], MMv[els], Fbv[els] = K, G, M, F
+    return KKv, GGv, MMv, Fbv
+
+
+def el2dof(EL):
+    """EL2DOF (nel, 30), 0-based dof 3*node+c."""
+    return (3 * EL[:, :, None] + np.arange(3)[None, None, :]).reshape(len(EL), 30)
+
+
+def global_assembly(mesh, KKv, GGv, MMv, Fbv):
+    """assembly_stokes_sph_3d_double_jacobian.m:409-480 (sparse3 path):
+    returns KK (tril CSC), GG (CSC), MM (tril CSC), Fb."""
+    EL = mesh.levels[0].EL2NOD
+    nU = 3 * mesh.nnod
+    nP = mesh.nVnod
+    D = el2dof(EL)
+    r, c = np.tril_indices(30)
+    order = np.lexsort((r, c))
+    r, c = r[order], c[order]
+    Ki, Kj = D[:, r], D[:, c]
+    sw = Ki < Kj
+    Ki2 = np.where(sw, Kj, Ki)
+    Kj2 = np.where(sw, Ki, Kj)
+    KK = sp.csc_matrix((KKv.ravel(), (Ki2.ravel(), Kj2.ravel())), shape=(nU, nU))
+    Gi = np.tile(D, (1, 4))
+    Gj = np.repeat(EL[:, :4], 30, axis=1)
+    GG = sp.csc_matrix((GGv.ravel(), (Gi.ravel(), Gj.ravel())), shape=(nU, nP))
+    Fb = np.bincount(D.ravel(), weights=Fbv.ravel(), minlength=nU)
+    rm, cm = np.tril_indices(4)
+    om = np.lexsort((rm, cm))
+    rm, cm = rm[om], cm[om]
+    Mi, Mj = EL[:, rm], EL[:, cm]
+    sw = Mi < Mj
+    Mi2 = np.where(sw, Mj, Mi)
+    Mj2 = np.where(sw, Mi, Mj)
+    MM = sp.csc_matrix((MMv.ravel(), (Mi2.ravel(), Mj2.ravel())), shape=(nP, nP))
+    return KK, GG, MM, Fb
+
+
+def assemble_opt(mesh, Visc, Dens, phys):
+    """Full 'opt' assembly -> (KK tril, Fb, GG, MM tril, Fpdil=0)."""
+    KKv, GGv, MMv, Fbv = element_matrices(mesh, Visc, Dens, phys)
+    KK, GG, MM, Fb = global_assembly(mesh, KKv, GGv, MMv, Fbv)
+    return KK, Fb, GG, MM, 0.0
+
+
+# --------------------------------------------------------------------------
+# 'std' restatement: B' D B with explicit rotation matrices
+# --------------------------------------------------------------------------
+
+D23 = np.zeros((6, 6))
+D23[:3, :3] = [[4 / 3, -2 / 3, -2 / 3], [-2 / 3, 4 / 3, -2 / 3], [-2 / 3, -2 / 3, 4 / 3]]
+D23[3, 3] = D23[4, 4] = D23[5, 5] = 1.0
+
+
+def class_kernel_std(kind, els, mesh, Visc, Dens, phys, T):
+    """Per-element math of the 'std' kernels, vectorised over elements.
+    Returns element-packed arrays in the same layout as the 'opt' kernels."""
+    EL = mesh.levels[0].EL2NOD[els]
+    n = len(els)
+    iso = kind == "in_cone_iso"
+    if kind in ("out_cone_no_cross_2pi", "out_cone_cross_2pi"):
+        th, ph, r = _vertex_sph(kind, mesh, EL)
+    elif kind == "in_cone_no_iso":
+        th, ph, r = E.elblk_cartesian2spherical(*_rot_x90(mesh.GCOORD[EL[:, :4]]))
+    else:
+        pos = {int(e): i for i, e in enumerate(mesh.classes["in_cone_iso"])}
+        X = mesh.X20_iso[[pos[int(e)] for e in els]]
+        th, ph, r = E.elblk_cartesian2spherical(*_rot_x90(X))
+    V = np.stack([th, ph, r], axis=1)           # (n, 3, nnode)
+    Kf = np.zeros((n, 30, 30))
+    Gf = np.zeros((n, 30, 4))
+    Mf = np.zeros((n, 4, 4))
+    Fb = np.zeros((n, 30))
+    for ip in range(len(T["IP_w"])):
+        NP = T["NP"][ip]
+        if iso:
+            J_SL = np.einsum("nk,emn->ekm", T["dNUds20"][ip], V)
+            pt = V @ T["NU20"][ip]
+        else:
+            J_SL = np.einsum("nk,emn->ekm", T["dN4ds"], V)
+            pt = V @ NP
+        th_ip, ph_ip, r_ip = pt[:, 0], pt[:, 1], pt[:, 2]
+        s_th, c_th, s_ph, c_ph = np.sin(th_ip), np.cos(th_ip), np.sin(ph_ip), np.cos(ph_ip)
+        detJ_CS = r_ip ** 2 * s_th
+        J_SC = np.stack([
+            np.stack([r_ip * s_th * c_th * c_ph, -r_ip * s_ph, r_ip ** 2 * s_th ** 2 * c_ph], 1),
+            np.stack([r_ip * s_th * c_th * s_ph, r_ip * c_ph, r_ip ** 2 * s_th ** 2 * s_ph], 1),
+            np.stack([-r_ip * s_th ** 2, np.zeros(n), r_ip ** 2 * s_th * c_th], 1)], 1) / detJ_CS[:, None, None]
+        J_dbl = J_SC @ np.linalg.inv(J_SL)
+        dNdx = J_dbl @ T["dNUds"][ip].T          # (n,3,10)
+        weight = detJ_CS * np.linalg.det(J_SL) * T["IP_w"][ip]
+        B = np.zeros((n, 6, 30))
+        B[:, 0, 0::3] = dNdx[:, 0]
+        B[:, 1, 1::3] = dNdx[:, 1]
+        B[:, 2, 2::3] = dNdx[:, 2]
+        B[:, 3, 1::3] = dNdx[:, 2]; B[:, 3, 2::3] = dNdx[:, 1]
+        B[:, 4, 0::3] = dNdx[:, 2]; B[:, 4, 2::3] = dNdx[:, 0]
+        B[:, 5, 0::3] = dNdx[:, 1]; B[:, 5, 1::3] = dNdx[:, 0]
+        visc = Visc[EL[:, :4]] @ NP
+        dens = Dens[EL[:, :4]] @ NP
+        Fg = phys["g"] * phys["Bscale"] * (dens - phys["Dens0"])
+        Kf += np.einsum("eai,ab,ebj->eij", B, D23, B) * (visc * weight)[:, None, None]
+        dvec = dNdx.transpose(0, 2, 1).reshape(n, 30)
+        Gf += dvec[:, :, None] * NP[None, None, :] * weight[:, None, None]
+        Mf += np.outer(NP, NP)[None] * (weight / visc)[:, None, None]
+        rhat = np.stack([s_th * c_ph, s_th * s_ph, c_th], 1)
+        Fb += (T["NU"][ip][None, :, None] * rhat[:, None, :]).reshape(n, 30) * (Fg * weight)[:, None]
+    # frame of the class -> original frame (rotation R maps original -> class)
+    if kind == "out_cone_cross_2pi":
+        R = np.diag([-1.0, -1.0, 1.0])
+    elif kind in ("in_cone_no_iso", "in_cone_iso"):
+        R = np.array([[1.0, 0, 0], [0, 0, -1.0], [0, 1.0, 0]])
+    else:
+        R = np.eye(3)
+    RR = np.kron(np.eye(10), R)
+    Kf = RR.T @ Kf @ RR
+    Gf = RR.T @ Gf
+    Fb = Fb @ RR
+    r_, c_ = np.tril_indices(30)
+    o = np.lexsort((r_, c_))
+    K = Kf[:, r_[o], c_[o]]
+    G = Gf.transpose(0, 2, 1).reshape(n, 120)
+    rm, cm = np.tril_indices(4)
+    om = np.lexsort((rm, cm))
+    M = Mf[:, rm[om], cm[om]]
+    return K, G, M, Fb
