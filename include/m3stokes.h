@@ -1,0 +1,181 @@
+/*
+ * m3stokes.h -- C ABI of the MI355X-native Stokes hot path of M3TET_SPH.
+ *
+ * The entry points replace, one for one, the MATLAB functions of the
+ * reference's Stokes path (paths relative to the reference repository):
+ *
+ *   m3_set_mesh            <- MESH fields read by the path
+ *                             (mfiles_SPH/init_mesh_curved_p.m:182-246)
+ *   m3_set_bc              <- UBC.ifix / UBC.vfix (SETUP_TEST/bc_sph.m:526-530)
+ *   m3_assemble            <- assembly_stokes_sph_3d_double_jacobian
+ *                             (mfiles_SPH/assembly_stokes_sph_3d_double_jacobian.m:1-2)
+ *                             followed by the rotation / BC elimination of
+ *                             mfiles_M3TET/flow3d_patera_mg_sph_p.m:225-298
+ *   m3_element_matrices    <- the element-packed ASSEMBLY.KKv/GGv/MMv/Fb arrays
+ *                             (assembly_stokes_sph_3d_double_jacobian.m:365-393)
+ *   m3_restrict_stiffmat   <- restrict_stiffmat (mfiles_M3TET/restrict_stiffmat.m:1)
+ *   m3_cholesky_factorization / m3_cholesky_solve
+ *                          <- cholesky_factorization.m:1 / cholesky_solve.m:1
+ *   m3_mgpcg               <- mgpcg_solver_p (mfiles_M3TET/mgpcg_solver_p.m:1-2)
+ *   m3_patera              <- flow3d_patera_mg_sph_p (flow3d_patera_mg_sph_p.m:1),
+ *                             pressure iteration and final velocity solve
+ *   m3_flow3d_patera_sph   <- flow3d_patera_sph (mfiles_M3TET/flow3d_patera_sph.m:1-9),
+ *                             the whole solve: assembly, RAP, factorisation, Patera
+ *
+ * Conventions: index arrays that mirror MATLAB arrays are 1-based and
+ * column-major exactly as MATLAB stores them (EL2NOD{l} is 10 x nel, so in C
+ * it is nel rows of 10).  Velocity dofs are node-interleaved, dof = 3*(node-1)+c
+ * (nod2dof.m:11); after rotation the components are (theta, phi, r).
+ * All pointers are host pointers owned by the caller and read only during the
+ * call; device state lives in the context until m3_destroy.  Every call is
+ * synchronous and returns 0 on success or a nonzero M3_E* code, with the
+ * message available from m3_last_error (the reference's error(...) sites).
+ * One host thread per context.
+ */
+#ifndef M3STOKES_H
+#define M3STOKES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3_MAX_LEVELS 8
+
+enum {
+  M3_OK = 0,
+  M3_E_ARG = 1,        /* invalid argument / inconsistent sizes */
+  M3_E_HIP = 2,        /* HIP runtime failure */
+  M3_E_MESH = 3,       /* mesh invariant violated (nesting, classes, ...) */
+  M3_E_NUMERIC = 4,    /* negative Jacobian, non-positive viscosity, NaN, ... */
+  M3_E_STATE = 5,      /* call order (e.g. solve before assemble) */
+  M3_E_NOTCONV = 6     /* iteration limit reached (result still returned) */
+};
+
+typedef struct m3_ctx m3_ctx;
+
+/* MESH fields used by the hot path.  Level 0 is the finest mesh (MATLAB's
+ * MESH.EL2NOD{1}); coarse-level nodes are numbered first in every finer
+ * level (tetmesh_p1_to_p2_sph.m:619-629). */
+typedef struct {
+  int32_t nmg;                               /* MESH.nmg, 2..M3_MAX_LEVELS */
+  int64_t nel[M3_MAX_LEVELS];                /* elements per level */
+  int64_t nnod[M3_MAX_LEVELS];               /* P2 nodes per level */
+  int64_t nVnod[M3_MAX_LEVELS];              /* vertices (pressure dofs) per level */
+  const uint32_t *EL2NOD[M3_MAX_LEVELS];     /* MESH.EL2NOD{l}: 10 x nel, 1-based */
+  const double *GCOORD;                      /* MESH.GCOORD: 3 x nnod (finest) */
+  const double *GCOORD_SPH;                  /* MESH.GCOORD_SPH: 3 x nnod */
+  const int32_t *els_class[4];               /* MESH.els_out_cone_no_cross_2pi,
+                                                els_out_cone_cross_2pi,
+                                                els_in_cone_no_iso,
+                                                els_in_cone_iso (1-based) */
+  int64_t nels_class[4];
+  const uint32_t *EL2NOD_cubic;              /* MESH.EL2NOD_cubic: 20 x n_iso, 1-based
+                                                into [GCOORD GCOORD_cubic] */
+  const double *GCOORD_cubic;                /* MESH.GCOORD_cubic: 3 x ncubic */
+  int64_t ncubic;
+  /* MESH.Ic2f{l} (fine level l <- coarse level l+1) at node level: the three
+   * velocity components share the weights (intermesh_transfer_matrix.m:50-55).
+   * CSR over fine nodes, 0-based coarse node columns. */
+  const int64_t *Ic2f_ptr[M3_MAX_LEVELS];
+  const int32_t *Ic2f_col[M3_MAX_LEVELS];
+  const double *Ic2f_val[M3_MAX_LEVELS];
+} m3_mesh_desc;
+
+/* PHYSICS / NUMSCALE / OPTS fields read by the assembly. */
+typedef struct {
+  double g;            /* PHYSICS.g */
+  double Bscale;       /* NUMSCALE.Bscale */
+  double Dens0;        /* NUMSCALE.Dens0 */
+  double minVisc;      /* PHYSICS.minVisc (used if has_minVisc) */
+  double maxVisc;      /* PHYSICS.maxVisc (used if has_maxVisc) */
+  int32_t has_minVisc;
+  int32_t has_maxVisc;
+  int32_t nip;         /* OPTS.nip, must be 14 */
+  int32_t type_Dmat;   /* 0: '23rd' (C1=4/3, C2=-2/3), 1: '221' (C1=2, C2=0) */
+} m3_phys;
+
+/* Precision of the multigrid V-cycle (the outer CG, dot products, Patera
+ * update and coarse Cholesky are FP64 in every mode). */
+enum { M3_PREC_FP64 = 0, M3_PREC_MIXED = 1 };
+
+/* Solver parameters: flow3d_patera_mg_sph_p.m:70-98, mgpcg_solver_p.m:674-682. */
+typedef struct {
+  double rtol_Pat;     /* 1e-4 */
+  double rtol_U;       /* 5e-3 */
+  int32_t itmax_U;     /* 20 */
+  int32_t itmin_Pat;   /* 3 */
+  int32_t itmax_Pat;   /* 120 */
+  int32_t pc_Pat;      /* 3 (only 3 is supported) */
+  double wght_MM;      /* 1 (one Jacobi sweep on MM) */
+  int32_t precision;   /* M3_PREC_FP64 or M3_PREC_MIXED */
+  int32_t warm_U;      /* 0: U starts from the BC values as in the reference
+                          (flow3d_patera_mg_sph_p.m:230); 1: use U as given */
+} m3_patera_opts;
+
+typedef struct {
+  int32_t nitP, nitU, nitZ;
+  double norm_rP;            /* final pressure residual (RMS) */
+  double norm_rP0;           /* initial pressure residual (RMS) */
+  double t_assemble, t_restrict, t_factor, t_pressure, t_velocity, t_total; /* s */
+} m3_info;
+
+int m3_create(m3_ctx **ctx, int hip_device);
+void m3_destroy(m3_ctx *ctx);
+const char *m3_last_error(const m3_ctx *ctx);
+
+int m3_set_mesh(m3_ctx *ctx, const m3_mesh_desc *mesh);
+/* ifix: 1-based rotated-frame dofs, vfix: values (UBC.ifix / UBC.vfix). */
+int m3_set_bc(m3_ctx *ctx, const int64_t *ifix, const double *vfix, int64_t nfix);
+
+/* Assembly on the device: K1 = RR' K RR with the BC dofs eliminated, GG = RR' GG,
+ * Fb = RR' Fb, MM, bc_rhs.  Visc/Dens are nodal (nnod). */
+int m3_assemble(m3_ctx *ctx, const double *Visc, const double *Dens, const m3_phys *phys);
+
+/* Element-packed matrices in the reference layout and original Cartesian
+ * frame: KKv 465 x nel, GGv 120 x nel, MMv 10 x nel, Fbv 30 x nel. */
+int m3_element_matrices(m3_ctx *ctx, const double *Visc, const double *Dens,
+                        const m3_phys *phys, double *KKv, double *GGv, double *MMv,
+                        double *Fbv);
+
+int m3_restrict_stiffmat(m3_ctx *ctx);
+int m3_cholesky_factorization(m3_ctx *ctx);
+/* Coarsest-level solve; rhs/x have 3*nnod[nmg-1] entries (fixed dofs ignored / 0). */
+int m3_cholesky_solve(m3_ctx *ctx, const double *rhs, double *x);
+
+/* MGPCG on the assembled operator: b, x have 3*nnod entries (rotated frame;
+ * entries at fixed dofs are ignored / returned as 0).  rrms_vec may be NULL,
+ * else it receives itmax+1 values. */
+int m3_mgpcg(m3_ctx *ctx, const double *b, double *x, double tol, int32_t is_abs_tol,
+             int32_t itmax, int32_t precision, int32_t *it, double *rrms_vec);
+
+/* Patera iteration on the assembled operators.  U: 3*nnod (rotated frame,
+ * output includes the BC values); P: nVnod (warm start in, solution out). */
+int m3_patera(m3_ctx *ctx, double *U, double *P, const m3_patera_opts *opts, m3_info *info);
+
+/* Whole solve: assemble, restrict, factorise, Patera. */
+int m3_flow3d_patera_sph(m3_ctx *ctx, const double *Visc, const double *Dens,
+                         const m3_phys *phys, double *U, double *P,
+                         const m3_patera_opts *opts, m3_info *info);
+
+/* Operator application for parity checks.  which: 0 = K on level `level`
+ * (level 0 = K1), 1 = GG (nVnod -> 3*nnod), 2 = GG' (3*nnod -> nVnod),
+ * 3 = MM (full symmetric), 4 = prolongation level+1 -> level (masked),
+ * 5 = restriction level -> level+1 (masked). */
+int m3_apply(m3_ctx *ctx, int32_t which, int32_t level, const double *x, double *y);
+
+/* Device vectors for parity checks.  which: 0 = Fb (rotated), 1 = bc_rhs,
+ * 2 = diag of K on `level`, 3 = free-dof mask on `level` (1/0),
+ * 4 = lumped MM. */
+int m3_get_vector(m3_ctx *ctx, int32_t which, int32_t level, double *out);
+
+/* Sizes: n[0]=levels, n[1..] per-level block nonzeros of K, then coarse factor
+ * statistics (free dofs, tiles, stored tiles, levels). */
+int m3_stats(m3_ctx *ctx, int64_t *out, int32_t nout);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,376 @@
+"""Python host mirror of the reference's Stokes-path functions over the C ABI.
+
+The functions keep the reference's names, argument meaning and error
+behaviour (``error(...)`` becomes ``M3Error``):
+
+* ``assembly_stokes_sph_3d_double_jacobian`` (mfiles_SPH/...double_jacobian.m:1-2)
+* ``restrict_stiffmat`` (mfiles_M3TET/restrict_stiffmat.m:1)
+* ``cholesky_factorization`` / ``cholesky_solve`` (cholesky_*.m:1)
+* ``mgpcg_solver_p`` (mfiles_M3TET/mgpcg_solver_p.m:1-2)
+* ``flow3d_patera_sph`` (mfiles_M3TET/flow3d_patera_sph.m:1-9)
+
+Matrices live on the GPU inside a :class:`StokesContext`; the functions that
+return MATLAB sparse matrices in the reference return SciPy sparse matrices
+gathered from the device.  Every compute call goes through ``libm3stokes.so``;
+if the library is missing the import of this module fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import time
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import mesh as meshmod
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libm3stokes.so")
+MAXL = 8
+
+
+class M3Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[m3 error {code}] {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m m3tet_sph_amd.build`")
+    return C.CDLL(LIB_PATH)
+
+
+LIB = _load()
+D = C.POINTER(C.c_double)
+I64 = C.POINTER(C.c_int64)
+I32 = C.POINTER(C.c_int32)
+U32 = C.POINTER(C.c_uint32)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nmg", C.c_int32), ("nel", C.c_int64 * MAXL), ("nnod", C.c_int64 * MAXL),
+                ("nVnod", C.c_int64 * MAXL), ("EL2NOD", U32 * MAXL), ("GCOORD", D),
+                ("GCOORD_SPH", D), ("els_class", I32 * 4), ("nels_class", C.c_int64 * 4),
+                ("EL2NOD_cubic", U32), ("GCOORD_cubic", D), ("ncubic", C.c_int64),
+                ("Ic2f_ptr", I64 * MAXL), ("Ic2f_col", I32 * MAXL), ("Ic2f_val", D * MAXL)]
+
+
+class Phys(C.Structure):
+    _fields_ = [("g", C.c_double), ("Bscale", C.c_double), ("Dens0", C.c_double),
+                ("minVisc", C.c_double), ("maxVisc", C.c_double), ("has_minVisc", C.c_int32),
+                ("has_maxVisc", C.c_int32), ("nip", C.c_int32), ("type_Dmat", C.c_int32)]
+
+
+class PateraOpts(C.Structure):
+    _fields_ = [("rtol_Pat", C.c_double), ("rtol_U", C.c_double), ("itmax_U", C.c_int32),
+                ("itmin_Pat", C.c_int32), ("itmax_Pat", C.c_int32), ("pc_Pat", C.c_int32),
+                ("wght_MM", C.c_double), ("precision", C.c_int32), ("warm_U", C.c_int32)]
+
+
+class Info(C.Structure):
+    _fields_ = [("nitP", C.c_int32), ("nitU", C.c_int32), ("nitZ", C.c_int32),
+                ("norm_rP", C.c_double), ("norm_rP0", C.c_double), ("t_assemble", C.c_double),
+                ("t_restrict", C.c_double), ("t_factor", C.c_double), ("t_pressure", C.c_double),
+                ("t_velocity", C.c_double), ("t_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+PREC = {"fp64": 0, "mixed": 1}
+
+LIB.m3_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+LIB.m3_destroy.argtypes = [C.c_void_p]
+LIB.m3_last_error.argtypes = [C.c_void_p]
+LIB.m3_last_error.restype = C.c_char_p
+LIB.m3_set_mesh.argtypes = [C.c_void_p, C.POINTER(MeshDesc)]
+LIB.m3_set_bc.argtypes = [C.c_void_p, I64, D, C.c_int64]
+LIB.m3_assemble.argtypes = [C.c_void_p, D, D, C.POINTER(Phys)]
+LIB.m3_element_matrices.argtypes = [C.c_void_p, D, D, C.POINTER(Phys), D, D, D, D]
+LIB.m3_restrict_stiffmat.argtypes = [C.c_void_p]
+LIB.m3_cholesky_factorization.argtypes = [C.c_void_p]
+LIB.m3_cholesky_solve.argtypes = [C.c_void_p, D, D]
+LIB.m3_mgpcg.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int32, C.c_int32, C.c_int32, I32, D]
+LIB.m3_patera.argtypes = [C.c_void_p, D, D, C.POINTER(PateraOpts), C.POINTER(Info)]
+LIB.m3_flow3d_patera_sph.argtypes = [C.c_void_p, D, D, C.POINTER(Phys), D, D,
+                                     C.POINTER(PateraOpts), C.POINTER(Info)]
+LIB.m3_apply.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D, D]
+LIB.m3_get_vector.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D]
+LIB.m3_stats.argtypes = [C.c_void_p, I64, C.c_int32]
+for _f in ("m3_create", "m3_set_mesh", "m3_set_bc", "m3_assemble", "m3_element_matrices",
+           "m3_restrict_stiffmat", "m3_cholesky_factorization", "m3_cholesky_solve", "m3_mgpcg",
+           "m3_patera", "m3_flow3d_patera_sph", "m3_apply", "m3_get_vector", "m3_stats"):
+    getattr(LIB, _f).restype = C.c_int
+
+
+def _p(a, t=D):
+    return a.ctypes.data_as(t)
+
+
+def phys_struct(phys, nip=14, type_Dmat="23rd"):
+    p = Phys()
+    p.g, p.Bscale, p.Dens0 = phys["g"], phys["Bscale"], phys["Dens0"]
+    p.has_minVisc = int(phys.get("minVisc") is not None)
+    p.has_maxVisc = int(phys.get("maxVisc") is not None)
+    p.minVisc = phys.get("minVisc") or 0.0
+    p.maxVisc = phys.get("maxVisc") or 0.0
+    p.nip = nip
+    p.type_Dmat = {"23rd": 0, "221": 1}[type_Dmat]
+    return p
+
+
+def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=120,
+                wght_MM=1.0, precision="fp64", warm_U=False):
+    """Defaults of flow3d_patera_mg_sph_p.m:70-98."""
+    o = PateraOpts()
+    o.rtol_Pat, o.rtol_U, o.itmax_U = rtol_Pat, rtol_U, itmax_U
+    o.itmin_Pat, o.itmax_Pat, o.pc_Pat, o.wght_MM = itmin_Pat, itmax_Pat, 3, wght_MM
+    o.precision = PREC[precision]
+    o.warm_U = int(warm_U)
+    return o
+
+
+class StokesContext:
+    """Device state of one mesh (patterns, operators, hierarchy, factor)."""
+
+    def __init__(self, mesh: meshmod.ShellMesh, device=0):
+        self._h = C.c_void_p()
+        rc = LIB.m3_create(C.byref(self._h), device)
+        if rc:
+            msg = LIB.m3_last_error(self._h).decode() if self._h else "m3_create failed"
+            if self._h:
+                LIB.m3_destroy(self._h)
+            self._h = None
+            raise M3Error(rc, msg)
+        self.mesh = mesh
+        self._keep = []
+        self._set_mesh(mesh)
+
+    def close(self):
+        if self._h:
+            LIB.m3_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise M3Error(rc, LIB.m3_last_error(self._h).decode())
+
+    def _set_mesh(self, m):
+        d = MeshDesc()
+        d.nmg = m.nmg
+        keep = []
+        for l, L in enumerate(m.levels):
+            d.nel[l], d.nnod[l], d.nVnod[l] = len(L.EL2NOD), L.nnod, L.nVnod
+            el = np.ascontiguousarray(L.EL2NOD + 1, dtype=np.uint32)
+            keep.append(el)
+            d.EL2NOD[l] = _p(el, U32)
+        X = np.ascontiguousarray(m.GCOORD, dtype=np.float64)
+        S = np.ascontiguousarray(m.GCOORD_SPH, dtype=np.float64)
+        keep += [X, S]
+        d.GCOORD, d.GCOORD_SPH = _p(X), _p(S)
+        names = ("out_cone_no_cross_2pi", "out_cone_cross_2pi", "in_cone_no_iso", "in_cone_iso")
+        for k, nm in enumerate(names):
+            a = np.ascontiguousarray(m.classes[nm] + 1, dtype=np.int32)
+            keep.append(a)
+            d.els_class[k] = _p(a, I32)
+            d.nels_class[k] = len(a)
+        iso = m.classes["in_cone_iso"]
+        n_iso = len(iso)
+        E20 = np.zeros((max(n_iso, 1), 20), dtype=np.uint32)
+        Xc = np.zeros((max(n_iso * 16, 1), 3))
+        if n_iso:
+            E20[:, :4] = m.levels[0].EL2NOD[iso, :4] + 1
+            E20[:, 4:] = m.nnod + 1 + np.arange(n_iso * 16).reshape(n_iso, 16)
+            Xc = np.ascontiguousarray(m.X20_iso[:, 4:, :].reshape(-1, 3))
+        keep += [E20, Xc]
+        d.EL2NOD_cubic, d.GCOORD_cubic, d.ncubic = _p(E20, U32), _p(Xc), n_iso * 16
+        for l in range(m.nmg - 1):
+            ip, ci, w = m.transfer(l)
+            ip = np.ascontiguousarray(ip, dtype=np.int64)
+            ci = np.ascontiguousarray(ci, dtype=np.int32)
+            w = np.ascontiguousarray(w, dtype=np.float64)
+            keep += [ip, ci, w]
+            d.Ic2f_ptr[l], d.Ic2f_col[l], d.Ic2f_val[l] = _p(ip, I64), _p(ci, I32), _p(w)
+        self._check(LIB.m3_set_mesh(self._h, C.byref(d)))
+
+    # ---- operations ------------------------------------------------------------
+    def set_bc(self, ifix0, vfix):
+        """UBC.ifix (0-based here, converted to 1-based) and UBC.vfix."""
+        i = np.ascontiguousarray(np.asarray(ifix0, dtype=np.int64) + 1)
+        v = np.ascontiguousarray(vfix, dtype=np.float64)
+        self._check(LIB.m3_set_bc(self._h, _p(i, I64), _p(v), len(i)))
+        self.ifix, self.vfix = np.asarray(ifix0), v
+
+    def assemble(self, Visc, Dens, phys, type_Dmat="23rd"):
+        V = np.ascontiguousarray(Visc, dtype=np.float64)
+        R = np.ascontiguousarray(Dens, dtype=np.float64)
+        self._check(LIB.m3_assemble(self._h, _p(V), _p(R), C.byref(phys_struct(phys, type_Dmat=type_Dmat))))
+
+    def element_matrices(self, Visc, Dens, phys, type_Dmat="23rd"):
+        nel = self.mesh.nel
+        K = np.zeros((nel, 465))
+        G = np.zeros((nel, 120))
+        M = np.zeros((nel, 10))
+        F = np.zeros((nel, 30))
+        V = np.ascontiguousarray(Visc, dtype=np.float64)
+        R = np.ascontiguousarray(Dens, dtype=np.float64)
+        self._check(LIB.m3_element_matrices(self._h, _p(V), _p(R),
+                                            C.byref(phys_struct(phys, type_Dmat=type_Dmat)),
+                                            _p(K), _p(G), _p(M), _p(F)))
+        return K, G, M, F
+
+    def restrict_stiffmat(self):
+        self._check(LIB.m3_restrict_stiffmat(self._h))
+
+    def cholesky_factorization(self):
+        self._check(LIB.m3_cholesky_factorization(self._h))
+
+    def cholesky_solve(self, rhs):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        self._check(LIB.m3_cholesky_solve(self._h, _p(rhs), _p(x)))
+        return x
+
+    def mgpcg(self, b, x=None, tol=1e-8, is_abs_tol=True, itmax=50, precision="fp64"):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x is None else np.ascontiguousarray(x, dtype=np.float64).copy()
+        it = C.c_int32(0)
+        hist = np.zeros(itmax + 1)
+        self._check(LIB.m3_mgpcg(self._h, _p(b), _p(x), tol, int(is_abs_tol), itmax, PREC[precision],
+                                 C.byref(it), _p(hist)))
+        return x, it.value, hist[:it.value + 1]
+
+    def patera(self, P0=None, U0=None, **opts):
+        n3 = 3 * self.mesh.nnod
+        U = np.zeros(n3) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).copy()
+        P = np.zeros(self.mesh.nVnod) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
+        info = Info()
+        o = patera_opts(warm_U=U0 is not None, **opts)
+        self._check(LIB.m3_patera(self._h, _p(U), _p(P), C.byref(o), C.byref(info)))
+        return U, P, info.as_dict()
+
+    def flow3d(self, Visc, Dens, phys, P0=None, U0=None, type_Dmat="23rd", **opts):
+        n3 = 3 * self.mesh.nnod
+        U = np.zeros(n3) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).copy()
+        P = np.zeros(self.mesh.nVnod) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
+        V = np.ascontiguousarray(Visc, dtype=np.float64)
+        R = np.ascontiguousarray(Dens, dtype=np.float64)
+        info = Info()
+        o = patera_opts(warm_U=U0 is not None, **opts)
+        self._check(LIB.m3_flow3d_patera_sph(self._h, _p(V), _p(R),
+                                             C.byref(phys_struct(phys, type_Dmat=type_Dmat)),
+                                             _p(U), _p(P), C.byref(o), C.byref(info)))
+        return U, P, info.as_dict()
+
+    def apply(self, which, level, x):
+        m = self.mesh
+        sizes = {0: 3 * m.levels[level].nnod, 1: 3 * m.nnod, 2: m.nVnod, 3: m.nVnod}
+        if which == 4:
+            ny = 3 * m.levels[level].nnod
+        elif which == 5:
+            ny = 3 * m.levels[level + 1].nnod
+        else:
+            ny = sizes[which]
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(ny)
+        self._check(LIB.m3_apply(self._h, which, level, _p(x), _p(y)))
+        return y
+
+    def get_vector(self, which, level=0):
+        m = self.mesh
+        n = m.nVnod if which == 4 else 3 * (m.levels[level].nnod if which in (2, 3) else m.nnod)
+        out = np.zeros(n)
+        self._check(LIB.m3_get_vector(self._h, which, level, _p(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(32, dtype=np.int64)
+        self._check(LIB.m3_stats(self._h, _p(out, I64), 32))
+        nmg = int(out[0])
+        return dict(nmg=nmg, nnzb=[int(v) for v in out[1:1 + nmg]],
+                    chol_n=int(out[1 + nmg]), chol_T=int(out[2 + nmg]), chol_tiles=int(out[3 + nmg]),
+                    chol_levels=int(out[4 + nmg]), chol_blevels=int(out[5 + nmg]),
+                    nslot=[int(v) for v in out[6 + nmg:6 + 2 * nmg]])
+
+
+# ---------------------------------------------------------------------------
+# reference-named mirrors
+# ---------------------------------------------------------------------------
+
+def _ctx(MESH):
+    ctx = getattr(MESH, "_m3ctx", None)
+    if ctx is None:
+        ctx = StokesContext(MESH)
+        MESH._m3ctx = ctx
+    return ctx
+
+
+def assembly_stokes_sph_3d_double_jacobian(VAR, MESH, PHYSICS, NUMSCALE, OPTS):
+    """[KK,Fb,GG,MM,Fpdil,DensEl,ViscIP] (assembly_stokes_sph_3d_double_jacobian.m:1-2).
+
+    Element matrices are computed on the GPU; the global sparse triplet
+    assembly (:398-480) is done here on the host.  KK and MM are returned
+    lower-triangular, as the reference does."""
+    if OPTS.get("nip", 14) != 14:
+        raise M3Error(1, "only OPTS.nip = 14 is supported")
+    phys = dict(g=PHYSICS["g"], Bscale=NUMSCALE["Bscale"], Dens0=NUMSCALE["Dens0"],
+                minVisc=PHYSICS.get("minVisc"), maxVisc=PHYSICS.get("maxVisc"))
+    ctx = _ctx(MESH)
+    KKv, GGv, MMv, Fbv = ctx.element_matrices(VAR["Visc"], VAR["Dens"], phys,
+                                              OPTS.get("type_Dmat", "23rd"))
+    EL = MESH.levels[0].EL2NOD
+    nU, nP = 3 * MESH.nnod, MESH.nVnod
+    dof = (3 * EL[:, :, None] + np.arange(3)).reshape(len(EL), 30)
+    r, c = np.tril_indices(30)
+    o = np.lexsort((r, c))
+    Ki, Kj = dof[:, r[o]], dof[:, c[o]]
+    KK = sp.csc_matrix((KKv.ravel(), (np.maximum(Ki, Kj).ravel(), np.minimum(Ki, Kj).ravel())),
+                       shape=(nU, nU))
+    GG = sp.csc_matrix((GGv.ravel(), (np.tile(dof, (1, 4)).ravel(), np.repeat(EL[:, :4], 30, 1).ravel())),
+                       shape=(nU, nP))
+    Fb = np.bincount(dof.ravel(), weights=Fbv.ravel(), minlength=nU)
+    rm, cm = np.tril_indices(4)
+    om = np.lexsort((rm, cm))
+    Mi, Mj = EL[:, rm[om]], EL[:, cm[om]]
+    MM = sp.csc_matrix((MMv.ravel(), (np.maximum(Mi, Mj).ravel(), np.minimum(Mi, Mj).ravel())),
+                       shape=(nP, nP))
+    return KK, Fb, GG, MM, 0.0, None, None
+
+
+def mgpcg_solver_p(ctx, b, x, OPTS):
+    """[x,itCG,time,rrms_vec] = mgpcg_solver_p(K1,K,KbD,permKbD,b,x,Ic2f_U,nmg,COMM,OPTS)
+    with the operators held by ``ctx`` (mgpcg_solver_p.m:1-2, options :704-713)."""
+    t0 = time.perf_counter()
+    x, it, hist = ctx.mgpcg(b, x, tol=OPTS.get("tol", 1e-8), is_abs_tol=OPTS.get("is_abs_tol", 1),
+                            itmax=OPTS.get("itmax", 50), precision=OPTS.get("precision", "fp64"))
+    return x, it, time.perf_counter() - t0, hist
+
+
+def flow3d_patera_sph(VAR, MESH, COMM, UBC, SETTINGS, PHYSICS, NUMSCALE, dt=None, **opts):
+    """[VAR,Uinfo] = flow3d_patera_sph(...) (flow3d_patera_sph.m:1-9): the MG
+    branch (nmg > 1) of flow3d_patera_mg_sph_p; returns Ux,Uy,Uz (Cartesian),
+    Uth,Uph,Ur (rotated) and P in VAR (flow3d_patera_mg_sph_p.m:672-681)."""
+    if MESH.nmg < 2:
+        raise M3Error(1, "only the multigrid branch (nmg > 1) is provided")
+    ctx = _ctx(MESH)
+    key = (id(UBC["ifix"]), len(UBC["ifix"]))
+    if getattr(ctx, "_bc_key", None) != key:
+        ctx.set_bc(UBC["ifix"], UBC["vfix"])
+        ctx._bc_key = key
+    phys = dict(g=PHYSICS["g"], Bscale=NUMSCALE["Bscale"], Dens0=NUMSCALE["Dens0"],
+                minVisc=PHYSICS.get("minVisc"), maxVisc=PHYSICS.get("maxVisc"))
+    U, P, info = ctx.flow3d(VAR["Visc"], VAR["Dens"], phys, P0=VAR.get("P"), **opts)
+    from .fields import rotation_frames
+    R = rotation_frames(MESH.GCOORD)
+    Uxyz = np.einsum("nij,nj->ni", R, U.reshape(-1, 3))
+    VAR = dict(VAR)
+    VAR.update(Uth=U[0::3].copy(), Uph=U[1::3].copy(), Ur=U[2::3].copy(), Ux=Uxyz[:, 0].copy(),
+               Uy=Uxyz[:, 1].copy(), Uz=Uxyz[:, 2].copy(), P=P)
+    return VAR, info

@@ -1,0 +1,56 @@
+"""Build libm3stokes.so for gfx950 in-tree with hipcc.
+
+Usage: ``python -m m3tet_sph_amd.build`` (also called by ``__graft_entry__.build``).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libm3stokes.so")
+SOURCES = ["m3_api.hip", "k_assembly.hip", "k_sparse.hip", "k_chol.hip"]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def build(verbose=True, jobs=4):
+    objs = []
+    procs = []
+    common = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-fopenmp", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+              "-Wno-unused-but-set-variable", "-Wno-unknown-pragmas"]
+    for src in SOURCES:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [HIPCC] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        while len([p for _, p in procs if p.poll() is None]) >= jobs:
+            procs[0][1].wait()
+    failed = False
+    for src, p in procs:
+        out = p.communicate()[0].decode(errors="replace")
+        out = "\n".join(l for l in out.splitlines()
+                        if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+        if p.returncode != 0:
+            failed = True
+            sys.stderr.write(f"[build] {src} failed\n{out}\n")
+        elif verbose and out.strip():
+            sys.stderr.write(f"[build] {src}:\n{out}\n")
+    if failed:
+        raise RuntimeError("hipcc compilation failed")
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-fopenmp", "-o", tmp] + objs
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    if verbose:
+        sys.stderr.write(f"[build] wrote {LIB}\n")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,665 @@
+// Element assembly of the double-Jacobian P2P1 Stokes operators and the
+// element-local Galerkin coarse operators, for gfx950.
+//
+// One wavefront per element.  Per integration point (14, Gellert & Harbord;
+// ip_tetrahedron.m:100-122) the wave builds the double Jacobian
+//   dN/dx = J_SC * inv(J_SL) * dN/dxi
+// (assembly_stokes_els_out_cone_no_cross_2pi_opt.m:93-164) in the frame of
+// the element's class, maps the gradients back to the original Cartesian
+// frame (the un-rotation of ..._cross_2pi_opt.m:385-429 and
+// ..._in_cone_no_iso_opt.m:390-496 applied to the gradient vectors instead of
+// to the packed matrices), and forms in LDS
+//   K  = sum_ip w eta (B' D B)      (30x30, '23rd' or '221' D matrix)
+//   G  = sum_ip w NP (dN/dx)        (30x4)
+//   M  = sum_ip (w/eta) NP NP'      (4x4)
+//   Fb = sum_ip w Fg r_hat N        (30)
+// It then applies the node rotations RR' . RR of flow3d_patera_mg_sph_p.m:
+// 233-243, lifts the prescribed velocities into bc_rhs (:280-283), removes
+// the fixed dofs (:297-298) and scatter-adds 3x3 blocks into the sliced
+// block-sparse operators.  The rotated, masked element matrix is kept for the
+// element-local Galerkin product of the next multigrid level.
+#include "m3_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int NIP = 14;
+
+__constant__ double c_ipw[NIP];
+__constant__ double c_N10[NIP][10];
+__constant__ double c_dN10[NIP][10][3];
+__constant__ double c_N4[NIP][4];
+__constant__ double c_N20[NIP][20];
+__constant__ double c_dN20[NIP][20][3];
+__constant__ double c_dN4[4][3];
+__constant__ uint16_t c_pq[465];  // packed-lower index -> (p | q<<8), column-major
+
+// ---- host restatement of the shape-function tables --------------------------
+
+void ip_rule(double ipx[3][NIP], double ipw[NIP]) {
+  // ip_tetrahedron.m:105-122 (degree 5, 14 points), weights / 6 (:189-191)
+  const double g = 1.0 / (46.0 * std::sqrt(46.0));
+  const double h = std::acos(g) + (2.0 / 3.0) * std::asin(g);
+  const double k = (104.0 + 8.0 * std::sqrt(46.0) * std::cos(h)) / 3.0;
+  const double s = std::sqrt(49.0 - k);
+  const double b = (7.0 + s) / k, a = 1.0 - 3.0 * b;
+  const double d = (7.0 - s) / k, c = 1.0 - 3.0 * d;
+  const double p = (98.0 - k - 14.0 * s) / (1680.0 * s * std::pow(b - a, 3));
+  const double q = (98.0 - k + 14.0 * s) / (1680.0 * s * std::pow(c - d, 3));
+  const double r = (1.0 - 4.0 * (p + q)) / 6.0;
+  const double e = (1.0 + std::pow(2.0 / (105.0 * r), 0.25)) / 4.0;
+  const double f = (1.0 - 2.0 * e) / 2.0;
+  const double X[3][NIP] = {{a, b, b, b, c, d, d, d, e, e, e, f, f, f},
+                            {b, a, b, b, d, c, d, d, e, f, f, e, e, f},
+                            {b, b, a, b, d, d, c, d, f, e, f, e, f, e}};
+  const double W[NIP] = {p, p, p, p, q, q, q, q, r, r, r, r, r, r};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < NIP; ++j) ipx[i][j] = X[i][j];
+  for (int j = 0; j < NIP; ++j) ipw[j] = W[j] / 6.0;
+}
+
+void sf10(double r, double s, double t, double N[10], double dN[10][3]) {
+  // 10-node tetrahedron, node order of sf_dsf_tet.m:829-840
+  const double u = 1.0 - r - s - t;
+  const double NN[10] = {r * (2 * r - 1), s * (2 * s - 1), t * (2 * t - 1), u * (2 * u - 1),
+                         4 * r * s, 4 * s * t, 4 * t * u, 4 * r * u, 4 * r * t, 4 * s * u};
+  const double D[10][3] = {{4 * r - 1, 0, 0}, {0, 4 * s - 1, 0}, {0, 0, 4 * t - 1},
+                           {1 - 4 * u, 1 - 4 * u, 1 - 4 * u}, {4 * s, 4 * r, 0},
+                           {0, 4 * t, 4 * s}, {-4 * t, -4 * t, 4 * (u - t)},
+                           {4 * (u - r), -4 * r, -4 * r}, {4 * t, 0, 4 * r},
+                           {-4 * s, 4 * (u - s), -4 * s}};
+  for (int i = 0; i < 10; ++i) {
+    N[i] = NN[i];
+    for (int j = 0; j < 3; ++j) dN[i][j] = D[i][j];
+  }
+}
+
+void sf20(double r, double s, double t, double N[20], double dN[20][3]) {
+  // 20-node (cubic) tetrahedron: vertices, two nodes per edge, face centres
+  const double u = 1.0 - r - s - t;
+  auto cv = [](double x) { return x * (3 * x - 2) * (1.5 * x - 0.5); };
+  auto ce = [](double x, double y, double z) { return x * y * (13.5 * z - 4.5); };
+  const double NN[20] = {cv(r), cv(s), cv(t), cv(u),
+                         ce(r, s, r), ce(r, s, s), ce(s, t, s), ce(s, t, t),
+                         ce(t, u, t), ce(t, u, u), ce(r, u, u), ce(r, u, r),
+                         ce(r, t, r), ce(r, t, t), ce(s, u, u), ce(s, u, s),
+                         27 * s * t * u, 27 * r * t * u, 27 * r * s * u, 27 * r * s * t};
+  // derivatives w.r.t. (r,s,t) with u = 1-r-s-t, by the product rule
+  auto dcv = [](double x) { return 13.5 * x * x - 9 * x + 1; };  // d/dx of cv
+  // edge function e(x,y,z=x or y) = x*y*(13.5 z - 4.5)
+  double D[20][3] = {};
+  D[0][0] = dcv(r);
+  D[1][1] = dcv(s);
+  D[2][2] = dcv(t);
+  for (int j = 0; j < 3; ++j) D[3][j] = -dcv(u);
+  // helper to accumulate derivative of x*y*(13.5 z - 4.5) where x,y,z are
+  // among {r,s,t,u}; grad of variable v: r->(1,0,0) ... u->(-1,-1,-1)
+  auto grad = [](int v, double g[3]) {
+    g[0] = g[1] = g[2] = 0;
+    if (v < 3) g[v] = 1;
+    else g[0] = g[1] = g[2] = -1;
+  };
+  const double var[4] = {r, s, t, u};
+  const int E[12][3] = {{0, 1, 0}, {0, 1, 1}, {1, 2, 1}, {1, 2, 2}, {2, 3, 2}, {2, 3, 3},
+                        {0, 3, 3}, {0, 3, 0}, {0, 2, 0}, {0, 2, 2}, {1, 3, 3}, {1, 3, 1}};
+  for (int k = 0; k < 12; ++k) {
+    const int a = E[k][0], b = E[k][1], z = E[k][2];
+    double ga[3], gb[3], gz[3];
+    grad(a, ga);
+    grad(b, gb);
+    grad(z, gz);
+    const double x = var[a], y = var[b], w = 13.5 * var[z] - 4.5;
+    for (int j = 0; j < 3; ++j)
+      D[4 + k][j] = ga[j] * y * w + x * gb[j] * w + x * y * 13.5 * gz[j];
+  }
+  const int F[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+  for (int k = 0; k < 4; ++k) {
+    const int a = F[k][0], b = F[k][1], z = F[k][2];
+    double ga[3], gb[3], gz[3];
+    grad(a, ga);
+    grad(b, gb);
+    grad(z, gz);
+    for (int j = 0; j < 3; ++j)
+      D[16 + k][j] = 27 * (ga[j] * var[b] * var[z] + var[a] * gb[j] * var[z] +
+                           var[a] * var[b] * gz[j]);
+  }
+  for (int i = 0; i < 20; ++i) {
+    N[i] = NN[i];
+    for (int j = 0; j < 3; ++j) dN[i][j] = D[i][j];
+  }
+}
+
+// ---- device helpers -----------------------------------------------------------
+
+__device__ inline int64_t sell_find(const int64_t *slice_ptr, const int32_t *rowlen,
+                                    const int32_t *col, int row, int colv) {
+  const int64_t base = slice_ptr[row >> 6] + (row & 63);
+  int lo = 0, hi = rowlen[row] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int c = col[base + (int64_t)mid * 64];
+    if (c == colv) return base + (int64_t)mid * 64;
+    if (c < colv) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+__device__ inline void cart2sph(double x, double y, double z, double &th, double &ph,
+                                double &r) {
+  r = sqrt(x * x + y * y + z * z);
+  th = atan2(sqrt(x * x + y * y), z);
+  ph = atan2(y, x);
+  if (ph < 0) ph += 2.0 * M_PI;
+}
+
+struct AsmArgs {
+  int64_t nel;
+  const int32_t *el2nod;
+  const uint8_t *cls;
+  const int32_t *iso_idx;
+  const double *X, *S, *X20, *frames, *visc, *dens, *mask, *U0;
+  double g, Bscale, Dens0, minV, maxV, C2;
+  int hasMin, hasMax;
+  SellView K, G, GT, M;
+  double *Fb, *bc_rhs, *Kel;
+  double *dbgK, *dbgG, *dbgM, *dbgF;
+  int write_global;
+  int *err;
+};
+
+__global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
+  const int64_t e = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ int s_nod[10];
+  __shared__ double s_J[NIP][9];      // rows: original-frame x_i; cols: d xi_k
+  __shared__ double s_wv[NIP], s_w[NIP], s_wd[NIP], s_wf[NIP], s_rh[NIP][3];
+  __shared__ double s_g[NIP][10][3];  // dN_n/dx_i at each IP (original frame)
+  __shared__ double s_K[30][31];
+  __shared__ double s_Kr[30][31];
+  __shared__ double s_RR[10][9];
+  __shared__ double s_Gr[30][4];
+  __shared__ double s_F[30];
+  __shared__ double s_msk[30], s_u0[30];
+
+  const int cls = a.cls[e];
+  if (lane < 10) s_nod[lane] = a.el2nod[e * 10 + lane];
+  __syncthreads();
+  if (lane < 30) {
+    const int n = s_nod[lane / 3];
+    s_msk[lane] = a.mask[3 * (int64_t)n + lane % 3];
+    s_u0[lane] = a.U0[3 * (int64_t)n + lane % 3];
+  }
+  for (int t = lane; t < 90; t += 64) s_RR[t / 9][t % 9] = a.frames[(int64_t)s_nod[t / 9] * 9 + t % 9];
+
+  // ---------------- per-IP geometry (one lane per integration point)
+  if (lane < NIP) {
+    const int ip = lane;
+    double th[20], ph[20], rr[20];
+    int nn = 4;
+    if (cls == 3) {
+      nn = 20;
+      const double *x20 = a.X20 + (int64_t)a.iso_idx[e] * 60;
+      for (int n = 0; n < 20; ++n) {
+        // 90-degree rotation about X: x'=x, y'=-z, z'=y
+        cart2sph(x20[3 * n], -x20[3 * n + 2], x20[3 * n + 1], th[n], ph[n], rr[n]);
+      }
+    } else if (cls == 2) {
+      for (int n = 0; n < 4; ++n) {
+        const double *x = a.X + 3 * (int64_t)s_nod[n];
+        cart2sph(x[0], -x[2], x[1], th[n], ph[n], rr[n]);
+      }
+    } else {
+      for (int n = 0; n < 4; ++n) {
+        const double *s = a.S + 3 * (int64_t)s_nod[n];
+        th[n] = s[0];
+        ph[n] = s[1];
+        rr[n] = s[2];
+        if (cls == 1) {  // 180-degree rotation about Z
+          ph[n] += M_PI;
+          if (ph[n] > 2.0 * M_PI) ph[n] -= 2.0 * M_PI;
+        }
+      }
+    }
+    // J_SL[k][m] = sum_n dN_n/dxi_k * X_m(n)
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double thi = 0, phi = 0, ri = 0;
+    if (nn == 20) {
+      for (int n = 0; n < 20; ++n) {
+        for (int k = 0; k < 3; ++k) {
+          const double dn = c_dN20[ip][n][k];
+          J[k][0] += dn * th[n];
+          J[k][1] += dn * ph[n];
+          J[k][2] += dn * rr[n];
+        }
+        thi += c_N20[ip][n] * th[n];
+        phi += c_N20[ip][n] * ph[n];
+        ri += c_N20[ip][n] * rr[n];
+      }
+    } else {
+      for (int n = 0; n < 4; ++n) {
+        for (int k = 0; k < 3; ++k) {
+          const double dn = c_dN4[n][k];
+          J[k][0] += dn * th[n];
+          J[k][1] += dn * ph[n];
+          J[k][2] += dn * rr[n];
+        }
+        thi += c_N4[ip][n] * th[n];
+        phi += c_N4[ip][n] * ph[n];
+        ri += c_N4[ip][n] * rr[n];
+      }
+    }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (det < 0) atomicOr(a.err, 1);
+    if (thi == 0.0) atomicOr(a.err, 2);
+    const double id = 1.0 / det;
+    // inv(J_SL)[m][k] = d xi_k / d sph_m
+    double I[3][3];
+    I[0][0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+    I[0][1] = id * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+    I[0][2] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    I[1][0] = id * (J[1][2] * J[2][0] - J[1][0] * J[2][2]);
+    I[1][1] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    I[1][2] = id * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+    I[2][0] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    I[2][1] = id * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+    I[2][2] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    // note: I is inv of the matrix with rows k; I[m][k] indexes (row m, col k)
+    const double st = sin(thi), ct = cos(thi), sp = sin(phi), cp = cos(phi);
+    const double dCS = ri * ri * st;
+    const double idCS = 1.0 / dCS;
+    // J_SC[i][m] = d sph_m / d x_i (class frame)
+    const double SC[3][3] = {{ri * st * ct * cp * idCS, -ri * sp * idCS, ri * ri * st * st * cp * idCS},
+                             {ri * st * ct * sp * idCS, ri * cp * idCS, ri * ri * st * st * sp * idCS},
+                             {-ri * st * st * idCS, 0.0, ri * ri * st * ct * idCS}};
+    double D[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) {
+        double acc = 0;
+        for (int m = 0; m < 3; ++m) acc += SC[i][m] * I[m][k];
+        D[i][k] = acc;
+      }
+    double rh[3] = {st * cp, st * sp, ct};
+    // class frame -> original frame for gradient rows and r_hat
+    double Do[3][3], rho[3];
+    if (cls == 1) {
+      for (int k = 0; k < 3; ++k) { Do[0][k] = -D[0][k]; Do[1][k] = -D[1][k]; Do[2][k] = D[2][k]; }
+      rho[0] = -rh[0]; rho[1] = -rh[1]; rho[2] = rh[2];
+    } else if (cls >= 2) {
+      for (int k = 0; k < 3; ++k) { Do[0][k] = D[0][k]; Do[1][k] = D[2][k]; Do[2][k] = -D[1][k]; }
+      rho[0] = rh[0]; rho[1] = rh[2]; rho[2] = -rh[1];
+    } else {
+      for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) Do[i][k] = D[i][k];
+      rho[0] = rh[0]; rho[1] = rh[1]; rho[2] = rh[2];
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) s_J[ip][3 * i + k] = Do[i][k];
+    const double w = dCS * det * c_ipw[ip];
+    double vi = 0, di = 0;
+    for (int n = 0; n < 4; ++n) {
+      vi += a.visc[s_nod[n]] * c_N4[ip][n];
+      di += a.dens[s_nod[n]] * c_N4[ip][n];
+    }
+    if (a.hasMin) vi = fmax(vi, a.minV);
+    if (a.hasMax) vi = fmin(vi, a.maxV);
+    if (!(vi > 0)) atomicOr(a.err, 4);
+    const double Fg = a.g * a.Bscale * (di - a.Dens0);
+    s_w[ip] = w;
+    s_wv[ip] = w * vi;
+    s_wd[ip] = w / vi;
+    s_wf[ip] = w * Fg;
+    s_rh[ip][0] = rho[0];
+    s_rh[ip][1] = rho[1];
+    s_rh[ip][2] = rho[2];
+  }
+  __syncthreads();
+  // ---------------- gradients in the original frame
+  for (int t = lane; t < NIP * 10; t += 64) {
+    const int ip = t / 10, n = t % 10;
+    for (int i = 0; i < 3; ++i)
+      s_g[ip][n][i] = s_J[ip][3 * i] * c_dN10[ip][n][0] + s_J[ip][3 * i + 1] * c_dN10[ip][n][1] +
+                      s_J[ip][3 * i + 2] * c_dN10[ip][n][2];
+  }
+  __syncthreads();
+  // ---------------- K (original frame): K[(a,c)][(b,d)] =
+  //   sum_ip w eta [delta_cd g_a.g_b + g_b[c] g_a[d] + C2 g_b[d] g_a[c]]
+  const double C2 = a.C2;
+  for (int t = lane; t < 465; t += 64) {
+    const int pq = c_pq[t];
+    const int p = pq & 255, q = pq >> 8;
+    const int na = p / 3, c = p % 3, nb = q / 3, d = q % 3;
+    double acc = 0;
+    for (int ip = 0; ip < NIP; ++ip) {
+      const double *ga = s_g[ip][na];
+      const double *gb = s_g[ip][nb];
+      double v = gb[c] * ga[d] + C2 * gb[d] * ga[c];
+      if (c == d) v += ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+      acc += s_wv[ip] * v;
+    }
+    s_K[p][q] = acc;
+    s_K[q][p] = acc;
+    if (a.dbgK) a.dbgK[e * 465 + t] = acc;
+  }
+  // ---------------- G (original frame), M, Fb
+  for (int t = lane; t < 120; t += 64) {
+    const int v = t / 30, p = t % 30, na = p / 3, c = p % 3;
+    double acc = 0;
+    for (int ip = 0; ip < NIP; ++ip) acc += s_w[ip] * c_N4[ip][v] * s_g[ip][na][c];
+    s_Gr[p][v] = acc;
+    if (a.dbgG) a.dbgG[e * 120 + t] = acc;
+  }
+  if (lane < 30) {
+    const int na = lane / 3, c = lane % 3;
+    double acc = 0;
+    for (int ip = 0; ip < NIP; ++ip) acc += s_wf[ip] * c_N10[ip][na] * s_rh[ip][c];
+    s_F[lane] = acc;
+    if (a.dbgF) a.dbgF[e * 30 + lane] = acc;
+  }
+  double mval = 0;
+  int mv = 0, mw = 0;
+  if (lane >= 32 && lane < 48) {
+    mv = (lane - 32) / 4;
+    mw = (lane - 32) % 4;
+    for (int ip = 0; ip < NIP; ++ip) mval += s_wd[ip] * c_N4[ip][mv] * c_N4[ip][mw];
+    if (a.dbgM && mv >= mw) {
+      // packed lower column-major: column mw, row mv
+      const int idx = mw * 4 - mw * (mw - 1) / 2 + (mv - mw);
+      a.dbgM[e * 10 + idx] = mval;
+    }
+  }
+  __syncthreads();
+  if (!a.write_global) return;
+  // ---------------- rotate blocks to the nodal (theta, phi, r) frames
+  for (int t = lane; t < 100; t += 64) {
+    const int na = t / 10, nb = t % 10;
+    double T1[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int dd = 0; dd < 3; ++dd) {
+        double acc = 0;
+        for (int j = 0; j < 3; ++j) acc += s_K[3 * na + i][3 * nb + j] * s_RR[nb][3 * j + dd];
+        T1[i][dd] = acc;
+      }
+    for (int cc = 0; cc < 3; ++cc)
+      for (int dd = 0; dd < 3; ++dd) {
+        double acc = 0;
+        for (int i = 0; i < 3; ++i) acc += s_RR[na][3 * i + cc] * T1[i][dd];
+        s_Kr[3 * na + cc][3 * nb + dd] = acc;
+      }
+  }
+  __syncthreads();
+  // ---------------- BC lift (bc_rhs = K U0) and removal of fixed dofs
+  if (lane < 30) {
+    double acc = 0;
+    for (int q = 0; q < 30; ++q) acc += s_Kr[lane][q] * s_u0[q];
+    if (acc != 0.0) atomicAdd(&a.bc_rhs[3 * (int64_t)s_nod[lane / 3] + lane % 3], acc);
+  }
+  __syncthreads();
+  for (int t = lane; t < 900; t += 64) {
+    const int p = t / 30, q = t % 30;
+    s_Kr[p][q] *= s_msk[p] * s_msk[q];
+  }
+  __syncthreads();
+  if (a.Kel) {
+    for (int t = lane; t < 465; t += 64) {
+      const int pq = c_pq[t];
+      a.Kel[e * 465 + t] = s_Kr[pq & 255][pq >> 8];
+    }
+  }
+  // ---------------- scatter K blocks
+  for (int t = lane; t < 100; t += 64) {
+    const int na = t / 10, nb = t % 10;
+    const int64_t pos = sell_find(a.K.slice_ptr, a.K.rowlen, a.K.col, s_nod[na], s_nod[nb]);
+    if (pos < 0) { atomicOr(a.err, 8); continue; }
+    for (int cc = 0; cc < 3; ++cc)
+      for (int dd = 0; dd < 3; ++dd)
+        atomicAdd(a.K.val + (3 * cc + dd) * a.K.nslot + pos, s_Kr[3 * na + cc][3 * nb + dd]);
+  }
+  // ---------------- G (rotated): G_rot[a,:,v] = RR_a' G[a,:,v]
+  if (lane < 40) {
+    const int na = lane / 4, v = lane % 4;
+    double gr[3];
+    for (int cc = 0; cc < 3; ++cc) {
+      double acc = 0;
+      for (int i = 0; i < 3; ++i) acc += s_RR[na][3 * i + cc] * s_Gr[3 * na + i][v];
+      gr[cc] = acc;
+    }
+    const int node = s_nod[na], vert = s_nod[v];
+    const int64_t p1 = sell_find(a.G.slice_ptr, a.G.rowlen, a.G.col, node, vert);
+    const int64_t p2 = sell_find(a.GT.slice_ptr, a.GT.rowlen, a.GT.col, vert, node);
+    if (p1 < 0 || p2 < 0) atomicOr(a.err, 8);
+    else
+      for (int cc = 0; cc < 3; ++cc) {
+        atomicAdd(a.G.val + cc * a.G.nslot + p1, gr[cc]);
+        atomicAdd(a.GT.val + cc * a.GT.nslot + p2, gr[cc]);
+      }
+  }
+  if (lane >= 32 && lane < 48) {
+    const int64_t p = sell_find(a.M.slice_ptr, a.M.rowlen, a.M.col, s_nod[mv], s_nod[mw]);
+    if (p < 0) atomicOr(a.err, 8);
+    else atomicAdd(a.M.val + p, mval);
+  }
+  // ---------------- Fb (rotated)
+  if (lane < 30) {
+    const int na = lane / 3, cc = lane % 3;
+    double acc = 0;
+    for (int i = 0; i < 3; ++i) acc += s_RR[na][3 * i + cc] * s_F[3 * na + i];
+    atomicAdd(&a.Fb[3 * (int64_t)s_nod[na] + cc], acc);
+  }
+}
+
+// ---- element-local Galerkin product --------------------------------------------
+// Coarse element c collects its 8 children e = 8c+k (tetmesh_p2_to_p1.m:112):
+//   Kc = mask_c . sum_k (Q_k (x) I3)' K_e (Q_k (x) I3) . mask_c
+// which equals Ic2f_U' K Ic2f_U (restrict_stiffmat.m:40-49) because every fine
+// node of a child lies in the closure of its parent, where the P2
+// interpolation rows of Ic2f are the parent's shape functions.
+struct RapArgs {
+  int64_t nel_c;
+  const double *Kf;        // fine element matrices (packed 465)
+  const int32_t *ctype;    // fine element -> Q table
+  const double *Qtab;      // ntypes * 100, Q[i][a] = weight of coarse local a at fine local i
+  const int32_t *el2nod_c; // coarse connectivity
+  const double *mask_c;
+  double *Kc;              // coarse element matrices (packed 465)
+  SellView K;              // coarse operator
+  int *err;
+};
+
+__global__ __launch_bounds__(64) void k_rap(RapArgs a) {
+  const int64_t c = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ double s_Kf[30][31];
+  __shared__ double s_T[30][31];
+  __shared__ double s_Kc[30][31];
+  __shared__ double s_Q[10][10];
+  __shared__ int s_nod[10];
+  __shared__ double s_m[30];
+  if (lane < 10) s_nod[lane] = a.el2nod_c[c * 10 + lane];
+  for (int t = lane; t < 900; t += 64) s_Kc[t / 30][t % 30] = 0.0;
+  __syncthreads();
+  if (lane < 30) s_m[lane] = a.mask_c[3 * (int64_t)s_nod[lane / 3] + lane % 3];
+  for (int k = 0; k < 8; ++k) {
+    const int64_t e = 8 * c + k;
+    for (int t = lane; t < 465; t += 64) {
+      const int pq = c_pq[t];
+      const double v = a.Kf[e * 465 + t];
+      s_Kf[pq & 255][pq >> 8] = v;
+      s_Kf[pq >> 8][pq & 255] = v;
+    }
+    const double *Q = a.Qtab + (int64_t)a.ctype[e] * 100;
+    for (int t = lane; t < 100; t += 64) s_Q[t / 10][t % 10] = Q[t];
+    __syncthreads();
+    // T[p][(b,d)] = sum_j Kf[p][(j,d)] Q[j][b]
+    for (int t = lane; t < 900; t += 64) {
+      const int p = t / 30, q = t % 30, b = q / 3, d = q % 3;
+      double acc = 0;
+      for (int j = 0; j < 10; ++j) acc += s_Kf[p][3 * j + d] * s_Q[j][b];
+      s_T[p][q] = acc;
+    }
+    __syncthreads();
+    // Kc[(a,c)][q] += sum_i Q[i][a] T[(i,c)][q]
+    for (int t = lane; t < 900; t += 64) {
+      const int p = t / 30, q = t % 30, aa = p / 3, cc = p % 3;
+      double acc = 0;
+      for (int i = 0; i < 10; ++i) acc += s_Q[i][aa] * s_T[3 * i + cc][q];
+      s_Kc[p][q] += acc;
+    }
+    __syncthreads();
+  }
+  for (int t = lane; t < 900; t += 64) {
+    const int p = t / 30, q = t % 30;
+    s_Kc[p][q] *= s_m[p] * s_m[q];
+  }
+  __syncthreads();
+  if (a.Kc) {
+    for (int t = lane; t < 465; t += 64) {
+      const int pq = c_pq[t];
+      // symmetrise the rounding of the two triangles
+      a.Kc[c * 465 + t] = 0.5 * (s_Kc[pq & 255][pq >> 8] + s_Kc[pq >> 8][pq & 255]);
+    }
+  }
+  for (int t = lane; t < 100; t += 64) {
+    const int na = t / 10, nb = t % 10;
+    const int64_t pos = sell_find(a.K.slice_ptr, a.K.rowlen, a.K.col, s_nod[na], s_nod[nb]);
+    if (pos < 0) { atomicOr(a.err, 8); continue; }
+    for (int cc = 0; cc < 3; ++cc)
+      for (int dd = 0; dd < 3; ++dd) {
+        const double v = 0.5 * (s_Kc[3 * na + cc][3 * nb + dd] + s_Kc[3 * nb + dd][3 * na + cc]);
+        atomicAdd(a.K.val + (3 * cc + dd) * a.K.nslot + pos, v);
+      }
+  }
+}
+
+// identity at fixed dofs on the diagonal blocks, inverse diagonal
+__global__ void k_fix_diag(SellView K, const double *mask, int64_t nnod, double *dinv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nnod) return;
+  const int64_t pos = sell_find(K.slice_ptr, K.rowlen, K.col, (int)i, (int)i);
+  if (pos < 0) return;
+  for (int c = 0; c < 3; ++c) {
+    double *b = K.val + (4 * c) * K.nslot + pos;
+    if (mask[3 * i + c] == 0.0) *b = 1.0;
+    dinv[3 * i + c] = 1.0 / *b;
+  }
+}
+
+__global__ void k_lumped(SellView M, int64_t n, double *LiMM) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t base = M.slice_ptr[i >> 6] + (i & 63);
+  const int len = M.rowlen[i];
+  double s = 0;
+  for (int k = 0; k < len; ++k) s += M.val[base + (int64_t)k * 64];
+  LiMM[i] = 1.0 / s;
+}
+
+}  // namespace
+
+namespace m3 {
+
+void upload_sf_tables() {
+  double ipx[3][NIP], ipw[NIP];
+  ip_rule(ipx, ipw);
+  double N10[NIP][10], dN10[NIP][10][3], N4[NIP][4], N20[NIP][20], dN20[NIP][20][3];
+  for (int ip = 0; ip < NIP; ++ip) {
+    const double r = ipx[0][ip], s = ipx[1][ip], t = ipx[2][ip];
+    sf10(r, s, t, N10[ip], dN10[ip]);
+    N4[ip][0] = r;
+    N4[ip][1] = s;
+    N4[ip][2] = t;
+    N4[ip][3] = 1.0 - r - s - t;
+    sf20(r, s, t, N20[ip], dN20[ip]);
+  }
+  const double dN4[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
+  uint16_t pq[465];
+  int idx = 0;
+  for (int q = 0; q < 30; ++q)
+    for (int p = q; p < 30; ++p) pq[idx++] = (uint16_t)(p | (q << 8));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ipw), ipw, sizeof(ipw)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N10), N10, sizeof(N10)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN10), dN10, sizeof(dN10)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N4), N4, sizeof(N4)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N20), N20, sizeof(N20)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20), dN20, sizeof(dN20)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN4), dN4, sizeof(dN4)));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_pq), pq, sizeof(pq)));
+}
+
+void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double *dbgF,
+                     bool write_global) {
+  AsmArgs a{};
+  LevelDev &f = c->L[0];
+  a.nel = f.nel;
+  a.el2nod = f.el2nod.p;
+  a.cls = c->cls.p;
+  a.iso_idx = c->iso_idx.p;
+  a.X = c->X.p;
+  a.S = c->S.p;
+  a.X20 = c->X20.p;
+  a.frames = c->frames.p;
+  a.visc = c->visc.p;
+  a.dens = c->dens.p;
+  a.mask = f.mask.p;
+  a.U0 = c->U0.p;
+  a.g = c->phys.g;
+  a.Bscale = c->phys.Bscale;
+  a.Dens0 = c->phys.Dens0;
+  a.minV = c->phys.minVisc;
+  a.maxV = c->phys.maxVisc;
+  a.hasMin = c->phys.has_minVisc;
+  a.hasMax = c->phys.has_maxVisc;
+  a.C2 = c->phys.type_Dmat == 0 ? -2.0 / 3.0 : 0.0;
+  a.K = f.K.view();
+  a.G = c->G.view();
+  a.GT = c->GT.view();
+  a.M = c->M.view();
+  a.Fb = c->Fb.p;
+  a.bc_rhs = c->bc_rhs.p;
+  a.Kel = write_global ? f.Kel.p : nullptr;
+  a.dbgK = dbgK;
+  a.dbgG = dbgG;
+  a.dbgM = dbgM;
+  a.dbgF = dbgF;
+  a.write_global = write_global ? 1 : 0;
+  a.err = c->errflag.p;
+  hipLaunchKernelGGL(k_assemble, dim3((unsigned)f.nel), dim3(64), 0, c->stream, a);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_rap(m3_ctx *c, int l) {
+  LevelDev &f = c->L[l];
+  LevelDev &cl = c->L[l + 1];
+  RapArgs a{};
+  a.nel_c = cl.nel;
+  a.Kf = f.Kel.p;
+  a.ctype = f.child_type.p;
+  a.Qtab = f.Qtab.p;
+  a.el2nod_c = cl.el2nod.p;
+  a.mask_c = cl.mask.p;
+  a.Kc = (l + 1 < c->nmg - 1) ? cl.Kel.p : nullptr;
+  a.K = cl.K.view();
+  a.err = c->errflag.p;
+  hipLaunchKernelGGL(k_rap, dim3((unsigned)cl.nel), dim3(64), 0, c->stream, a);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_fix_diag(m3_ctx *c, int l) {
+  LevelDev &lv = c->L[l];
+  const int bs = 256;
+  hipLaunchKernelGGL(k_fix_diag, dim3((unsigned)((lv.nnod + bs - 1) / bs)), dim3(bs), 0,
+                     c->stream, lv.K.view(), lv.mask.p, lv.nnod, lv.dinv.p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_lumped_mm(m3_ctx *c) {
+  const int bs = 256;
+  hipLaunchKernelGGL(k_lumped, dim3((unsigned)((c->nV + bs - 1) / bs)), dim3(bs), 0, c->stream,
+                     c->M.view(), c->nV, c->LiMM.p);
+  HIPCHECK(hipGetLastError());
+}
+
+}  // namespace m3

@@ -1,0 +1,685 @@
+// Bandwidth-bound kernels of the MGPCG velocity solve for gfx950:
+// sliced 3x3-block SpMV and the fused Jacobi smoother sweep, the nested P2
+// transfer operators, the V-cycle driver and the vector kernels of the CG
+// and Patera iterations.
+//
+// V-cycle (mgpcg_solver_p.m:388-596, 'Setup 3' weights :674-682).  On each
+// level the down path is d = w1 D^-1 r0 followed by three fused sweeps
+//   r1 = r0 - K d ;  d += w D^-1 r1
+// and the restriction r0' = Ic2f_U' r1; the up path is d = dd + Ic2f_U d'
+// followed by three fused sweeps.  Every sweep is one kernel that reads the
+// matrix once.  Vectors are full length (3*nnod) with zeros at the fixed
+// dofs; the operators carry an identity at those dofs, which makes every
+// iterate equal to the reference's reduced vectors padded with zeros.
+//
+// Precision: value type MT (double, float or bf16 bits) and vector type VT
+// (double or float) are template parameters; the FP64 path instantiates
+// MT=VT=double, the mixed path runs the whole V-cycle in FP32 with BF16
+// coarse-level matrix values.
+#include "m3_internal.h"
+
+#include <hip/hip_bf16.h>
+
+namespace {
+
+template <class T>
+__device__ inline float to_f(T v);
+template <>
+__device__ inline float to_f<float>(float v) { return v; }
+template <>
+__device__ inline float to_f<uint16_t>(uint16_t v) {
+  return __uint_as_float(((uint32_t)v) << 16);
+}
+
+template <class MT>
+__device__ inline double mval(const MT *p);
+template <>
+__device__ inline double mval<double>(const double *p) { return *p; }
+template <>
+__device__ inline double mval<float>(const float *p) { return (double)*p; }
+
+template <class MT, class VT>
+struct Acc;  // accumulation type: double for FP64, float otherwise
+template <>
+struct Acc<double, double> { using T = double; };
+template <>
+struct Acc<float, float> { using T = float; };
+template <>
+struct Acc<uint16_t, float> { using T = float; };
+
+template <class MT>
+__device__ inline auto ldv(const MT *p) {
+  if constexpr (std::is_same<MT, uint16_t>::value) return to_f<uint16_t>(*p);
+  else return *p;
+}
+
+// y = K x   (optionally fused: r = r0 - K x; d_out = x + w Dinv r; dot partial)
+// mode bits: 1 write y, 2 compute r = r0 - y and write r_out, 4 write d_out,
+// 8 accumulate partial dot(x, y) into part[blockIdx]
+template <class MT, class VT, int MODE>
+__global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const VT *__restrict__ x,
+                                                const VT *__restrict__ r0, const VT *__restrict__ dinv,
+                                                VT w, VT *__restrict__ y, VT *__restrict__ r_out,
+                                                VT *__restrict__ d_out, double *__restrict__ part) {
+  using A = typename Acc<MT, VT>::T;
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  A y0 = 0, y1 = 0, y2 = 0;
+  if (row < nrow) {
+    const int64_t base = K.slice_ptr[row >> 6] + (row & 63);
+    const int len = K.rowlen[row];
+    const int64_t ns = K.nslot;
+    const MT *v = K.val;
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = base + (int64_t)k * 64;
+      const int64_t cc = 3 * (int64_t)K.col[p];
+      const A x0 = (A)x[cc], x1 = (A)x[cc + 1], x2 = (A)x[cc + 2];
+      y0 += (A)ldv(v + p) * x0 + (A)ldv(v + ns + p) * x1 + (A)ldv(v + 2 * ns + p) * x2;
+      y1 += (A)ldv(v + 3 * ns + p) * x0 + (A)ldv(v + 4 * ns + p) * x1 + (A)ldv(v + 5 * ns + p) * x2;
+      y2 += (A)ldv(v + 6 * ns + p) * x0 + (A)ldv(v + 7 * ns + p) * x1 + (A)ldv(v + 8 * ns + p) * x2;
+    }
+    const int64_t o = 3 * row;
+    if (MODE & 1) { y[o] = (VT)y0; y[o + 1] = (VT)y1; y[o + 2] = (VT)y2; }
+    if (MODE & 2) {
+      const A q0 = (A)r0[o] - y0, q1 = (A)r0[o + 1] - y1, q2 = (A)r0[o + 2] - y2;
+      if (r_out) { r_out[o] = (VT)q0; r_out[o + 1] = (VT)q1; r_out[o + 2] = (VT)q2; }
+      if (MODE & 4) {
+        d_out[o] = (VT)((A)x[o] + (A)w * (A)dinv[o] * q0);
+        d_out[o + 1] = (VT)((A)x[o + 1] + (A)w * (A)dinv[o + 1] * q1);
+        d_out[o + 2] = (VT)((A)x[o + 2] + (A)w * (A)dinv[o + 2] * q2);
+      }
+    }
+  }
+  if (MODE & 8) {
+    double s = 0;
+    if (row < nrow) {
+      const int64_t o = 3 * row;
+      s = (double)x[o] * (double)y0 + (double)x[o + 1] * (double)y1 + (double)x[o + 2] * (double)y2;
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  }
+}
+
+// generic scalar-block SELL SpMV with bs values per block, rows with 3
+// (bs==3, G) or 1 output component(s)
+// G:  y[3*row+c] = mask * sum_k val[c][p] * x[col]        (3x1 blocks)
+// GT: y[row]     = sum_k sum_c val[c][p] * x[3*col+c]     (1x3 blocks)
+// M:  y[row]     = sum_k val[p] * x[col]
+__global__ void k_spmv_G(SellView G, int64_t nrow, const double *x, double *y, const double *mask) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= nrow) return;
+  const int64_t base = G.slice_ptr[row >> 6] + (row & 63);
+  const int len = G.rowlen[row];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = base + (int64_t)k * 64;
+    const double xv = x[G.col[p]];
+    a0 += G.val[p] * xv;
+    a1 += G.val[G.nslot + p] * xv;
+    a2 += G.val[2 * G.nslot + p] * xv;
+  }
+  const int64_t o = 3 * row;
+  if (mask) { a0 *= mask[o]; a1 *= mask[o + 1]; a2 *= mask[o + 2]; }
+  y[o] = a0;
+  y[o + 1] = a1;
+  y[o + 2] = a2;
+}
+
+__global__ void k_spmv_GT(SellView G, int64_t nrow, const double *x, double *y) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= nrow) return;
+  const int64_t base = G.slice_ptr[row >> 6] + (row & 63);
+  const int len = G.rowlen[row];
+  double a = 0;
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = base + (int64_t)k * 64;
+    const int64_t c3 = 3 * (int64_t)G.col[p];
+    a += G.val[p] * x[c3] + G.val[G.nslot + p] * x[c3 + 1] + G.val[2 * G.nslot + p] * x[c3 + 2];
+  }
+  y[row] = a;
+}
+
+__global__ void k_spmv_M(SellView M, int64_t nrow, const double *x, double *y) {
+  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (row >= nrow) return;
+  const int64_t base = M.slice_ptr[row >> 6] + (row & 63);
+  const int len = M.rowlen[row];
+  double a = 0;
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = base + (int64_t)k * 64;
+    a += M.val[p] * x[M.col[p]];
+  }
+  y[row] = a;
+}
+
+// prolongation with the add of the saved defect: df = dd + mask_f * (P dc)
+template <class VT, class WT>
+__global__ void k_prolong(int64_t nf, const int64_t *ptr, const int32_t *col, const WT *w,
+                          const double *mask, const VT *dc, const VT *dd, VT *df) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  VT a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int64_t c3 = 3 * (int64_t)col[k];
+    const VT wk = (VT)w[k];
+    a0 += wk * dc[c3];
+    a1 += wk * dc[c3 + 1];
+    a2 += wk * dc[c3 + 2];
+  }
+  const int64_t o = 3 * i;
+  df[o] = (dd ? dd[o] : (VT)0) + (VT)mask[o] * a0;
+  df[o + 1] = (dd ? dd[o + 1] : (VT)0) + (VT)mask[o + 1] * a1;
+  df[o + 2] = (dd ? dd[o + 2] : (VT)0) + (VT)mask[o + 2] * a2;
+}
+
+// restriction rc = mask_c * (P' rf); also writes the first smoother iterate
+// d1 = w1 Dinv rc of the coarse level when d1 != nullptr
+template <class VT, class WT>
+__global__ void k_restrict(int64_t nc, const int64_t *ptr, const int32_t *col, const WT *w,
+                           const double *mask, const VT *rf, VT *rc, const VT *dinv, VT w1, VT *d1) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j >= nc) return;
+  VT a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t k = ptr[j]; k < ptr[j + 1]; ++k) {
+    const int64_t f3 = 3 * (int64_t)col[k];
+    const VT wk = (VT)w[k];
+    a0 += wk * rf[f3];
+    a1 += wk * rf[f3 + 1];
+    a2 += wk * rf[f3 + 2];
+  }
+  const int64_t o = 3 * j;
+  a0 *= (VT)mask[o];
+  a1 *= (VT)mask[o + 1];
+  a2 *= (VT)mask[o + 2];
+  rc[o] = a0;
+  rc[o + 1] = a1;
+  rc[o + 2] = a2;
+  if (d1) {
+    d1[o] = w1 * dinv[o] * a0;
+    d1[o + 1] = w1 * dinv[o + 1] * a1;
+    d1[o + 2] = w1 * dinv[o + 2] * a2;
+  }
+}
+
+template <class VT>
+__global__ void k_scale(int64_t n, VT w, const VT *dinv, const VT *r, VT *d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) d[i] = w * dinv[i] * r[i];
+}
+
+template <class A, class B>
+__global__ void k_convert(int64_t n, const A *x, B *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (B)x[i];
+}
+
+__global__ void k_to_bf16(int64_t n, const double *x, uint16_t *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  // round to nearest even
+  const float f = (float)x[i];
+  uint32_t u = __float_as_uint(f);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  y[i] = (uint16_t)(u >> 16);
+}
+
+// ---- reductions (deterministic two-stage) -----------------------------------------
+constexpr int RB = 256;
+
+__global__ void k_dot(int64_t n, const double *a, const double *b, double *part) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    s += a[i] * b[i];
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// two dot products: s0 = (a - c).b, s1 = a.b
+__global__ void k_dot2(int64_t n, const double *a, const double *c, const double *b, double *part,
+                       int nb) {
+  double s0 = 0, s1 = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    const double bb = b[i];
+    s1 += a[i] * bb;
+    if (c) s0 += (a[i] - c[i]) * bb;
+  }
+  __shared__ double sh0[RB], sh1[RB];
+  sh0[threadIdx.x] = s0;
+  sh1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      sh0[threadIdx.x] += sh0[threadIdx.x + st];
+      sh1[threadIdx.x] += sh1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = sh0[0];
+    part[nb + blockIdx.x] = sh1[0];
+  }
+}
+
+__global__ void k_vsum(int64_t n, const double *a, double *part) {
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) s += a[i];
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum(const double *part, int nb, double *out) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// ---- CG scalar program ----------------------------------------------------------------
+// scal layout: 0 rd, 1 rdlast, 2 beta, 3 alpha, 4 qKq, 5 rr, 6 tmp0, 7 tmp1
+__global__ void k_cg_beta(const double *part, int nb, double *scal, int first) {
+  double s0 = 0, s1 = 0;
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    s0 += part[i];
+    s1 += part[nb + i];
+  }
+  __shared__ double sh0[RB], sh1[RB];
+  sh0[threadIdx.x] = s0;
+  sh1[threadIdx.x] = s1;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      sh0[threadIdx.x] += sh0[threadIdx.x + st];
+      sh1[threadIdx.x] += sh1[threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // beta = ((r - rlast) . d) / rdlast  (mgpcg_solver_p.m:161), rd = r.d (:166)
+    scal[1] = scal[0];
+    scal[2] = first ? 0.0 : sh0[0] / scal[1];
+    scal[0] = sh1[0];
+  }
+}
+
+__global__ void k_cg_alpha(const double *part, int nb, double *scal) {
+  double s = 0;
+  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    scal[4] = sh[0];
+    scal[3] = scal[0] / sh[0];  // alpha = rd / qKq (:173)
+  }
+}
+
+// q = d + beta q
+__global__ void k_cg_dir(int64_t n, const double *d, double *q, const double *scal) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) q[i] = d[i] + scal[2] * q[i];
+}
+
+// x += alpha q; rlast = r; r -= alpha Kq; partial |r|^2
+__global__ void k_cg_update(int64_t n, double *x, const double *q, double *r, double *rlast,
+                            const double *Kq, const double *scal, double *part) {
+  const double alpha = scal[3];
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    x[i] += alpha * q[i];
+    const double ri = r[i];
+    rlast[i] = ri;
+    const double rn = ri - alpha * Kq[i];
+    r[i] = rn;
+    s += rn * rn;
+  }
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_axpy(int64_t n, double a, const double *x, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+__global__ void k_xpby(int64_t n, const double *x, double b, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (x ? x[i] : 0.0) + b * y[i];
+}
+__global__ void k_copy(int64_t n, const double *x, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+__global__ void k_scale_mask(int64_t n, const double *x, const double *m, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] * m[i];
+}
+__global__ void k_add_scalar(int64_t n, double *a, double v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] += v;
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+namespace m3 {
+
+void spmv_K(hipStream_t s, LevelDev &lv, const double *x, double *y) {
+  hipLaunchKernelGGL((k_smooth<double, double, 1>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.K.view(),
+                     lv.nnod, x, nullptr, nullptr, 0.0, y, nullptr, nullptr, nullptr);
+}
+
+void spmv_G(hipStream_t s, Sell &G, const double *q, double *y, const double *mask) {
+  hipLaunchKernelGGL(k_spmv_G, dim3(nblk(G.nrow)), dim3(256), 0, s, G.view(), G.nrow, q, y, mask);
+}
+void spmv_GT(hipStream_t s, Sell &GT, const double *z, double *y) {
+  hipLaunchKernelGGL(k_spmv_GT, dim3(nblk(GT.nrow)), dim3(256), 0, s, GT.view(), GT.nrow, z, y);
+}
+void spmv_M(hipStream_t s, Sell &M, const double *x, double *y) {
+  hipLaunchKernelGGL(k_spmv_M, dim3(nblk(M.nrow)), dim3(256), 0, s, M.view(), M.nrow, x, y);
+}
+void prolong(hipStream_t s, LevelDev &fine, const double *dc, const double *dd, double *df) {
+  hipLaunchKernelGGL((k_prolong<double, double>), dim3(nblk(fine.nnod)), dim3(256), 0, s, fine.nnod,
+                     fine.P_ptr.p, fine.P_col.p, fine.P_val.p, fine.mask.p, dc, dd, df);
+}
+void restrict_(hipStream_t s, LevelDev &fine, LevelDev &coarse, const double *rf, double *rc) {
+  hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(coarse.nnod)), dim3(256), 0, s,
+                     coarse.nnod, fine.R_ptr.p, fine.R_col.p, fine.R_val.p, coarse.mask.p, rf, rc,
+                     (const double *)nullptr, 0.0, (double *)nullptr);
+}
+
+static void add_scalar_impl(hipStream_t s, double *a, double v, int64_t n) {
+  hipLaunchKernelGGL(k_add_scalar, dim3(nblk(n)), dim3(256), 0, s, n, a, v);
+}
+
+// --------------------------------------------------------------------------------
+// V-cycle.  FP64: vectors double, matrices double.  Mixed: the residual is
+// converted to FP32, the cycle runs in FP32 with FP32 values on the finest
+// two levels... (levels >= 1 use BF16 values when bf16 copies exist), and the
+// defect is converted back to FP64.
+// --------------------------------------------------------------------------------
+static const double WDOWN1[3] = {0.9, 0.4, 0.25}, WUP1[3] = {0.25, 0.4, 0.9};
+static const double WDOWN[3] = {0.25, 0.4, 0.9}, WUP[3] = {0.9, 0.4, 0.25};
+
+template <class VT>
+struct VecSet {
+  VT *r0, *r1, *d, *dn, *dd, *rr;
+  const VT *dinv;
+};
+
+static VecSet<double> vecs64(LevelDev &l) {
+  return VecSet<double>{l.r0.p, l.r1.p, l.d.p, l.dn.p, l.dd.p, l.rr.p, l.dinv.p};
+}
+static VecSet<float> vecs32(LevelDev &l) {
+  return VecSet<float>{l.r0f.p, l.r1f.p, l.df.p, l.dnf.p, l.ddf.p, l.rrf.p, l.dinv32.p};
+}
+
+template <class MT, class VT>
+static void sweep(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
+                  double w, VT *r_out, VT *d_out) {
+  if (d_out)
+    hipLaunchKernelGGL((k_smooth<MT, VT, 6>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
+                       (VT *)nullptr, r_out, d_out, (double *)nullptr);
+  else
+    hipLaunchKernelGGL((k_smooth<MT, VT, 2>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
+                       (VT *)nullptr, r_out, (VT *)nullptr, (double *)nullptr);
+}
+
+template <class VT, class Pick>
+static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
+  hipStream_t s = c->stream;
+  const int nm = c->nmg;
+  // r0 of level 0 already in place
+  for (int l = 0; l < nm - 1; ++l) {
+    LevelDev &lv = c->L[l];
+    VecSet<VT> v = vs(lv);
+    const double *wd = l == 0 ? WDOWN1 : WDOWN;
+    const int64_t n3 = 3 * lv.nnod;
+    if (l == 0)
+      hipLaunchKernelGGL((k_scale<VT>), dim3(nblk(n3)), dim3(256), 0, s, n3, (VT)wd[0], v.dinv, v.r0, v.d);
+    // d (= w1 Dinv r0) is in v.d; rr = r0 is kept (v.r0 not overwritten)
+    pick_K(l, [&](auto K) {
+      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wd[1], (VT *)nullptr, v.dn);
+      sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wd[2], (VT *)nullptr, v.dd);
+      sweep(s, K, lv.nnod, v.dd, v.r0, v.dinv, 0.0, v.r1, (VT *)nullptr);
+    });
+    LevelDev &cl = c->L[l + 1];
+    VecSet<VT> vc = vs(cl);
+    const bool coarsest = (l + 1 == nm - 1);
+    if constexpr (std::is_same<VT, double>::value)
+      hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(cl.nnod)), dim3(256), 0, s, cl.nnod,
+                         lv.R_ptr.p, lv.R_col.p, lv.R_val.p, cl.mask.p, v.r1, vc.r0,
+                         coarsest ? (const double *)nullptr : vc.dinv, WDOWN[0],
+                         coarsest ? (double *)nullptr : vc.d);
+    else
+      hipLaunchKernelGGL((k_restrict<float, float>), dim3(nblk(cl.nnod)), dim3(256), 0, s, cl.nnod,
+                         lv.R_ptr.p, lv.R_col.p, lv.R_val32.p, cl.mask.p, v.r1, vc.r0,
+                         coarsest ? (const float *)nullptr : vc.dinv, (float)WDOWN[0],
+                         coarsest ? (float *)nullptr : vc.d);
+  }
+  // coarsest: FP64 Cholesky
+  {
+    LevelDev &cl = c->L[nm - 1];
+    const int64_t n3 = 3 * cl.nnod;
+    if constexpr (std::is_same<VT, double>::value) {
+      chol_solve_dev(c, cl.r0.p, cl.d.p);
+    } else {
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(256), 0, s, n3, cl.r0f.p, cl.r0.p);
+      chol_solve_dev(c, cl.r0.p, cl.d.p);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, cl.d.p, cl.df.p);
+    }
+  }
+  for (int l = nm - 2; l >= 0; --l) {
+    LevelDev &lv = c->L[l];
+    LevelDev &cl = c->L[l + 1];
+    VecSet<VT> v = vs(lv);
+    VecSet<VT> vc = vs(cl);
+    const double *wu = l == 0 ? WUP1 : WUP;
+    if constexpr (std::is_same<VT, double>::value)
+      hipLaunchKernelGGL((k_prolong<double, double>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.nnod,
+                         lv.P_ptr.p, lv.P_col.p, lv.P_val.p, lv.mask.p, vc.d, v.dd, v.d);
+    else
+      hipLaunchKernelGGL((k_prolong<float, float>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.nnod,
+                         lv.P_ptr.p, lv.P_col.p, lv.P_val32.p, lv.mask.p, vc.d, v.dd, v.d);
+    pick_K(l, [&](auto K) {
+      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[0], (VT *)nullptr, v.dn);
+      sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[1], (VT *)nullptr, v.d);
+      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[2], (VT *)nullptr, v.dn);
+    });
+    // result of the last sweep is in dn: swap so that d holds it
+    if constexpr (std::is_same<VT, double>::value) std::swap(lv.d, lv.dn);
+    else std::swap(lv.df, lv.dnf);
+  }
+}
+
+void vcycle(m3_ctx *c, const double *r, double *d, int precision) {
+  hipStream_t s = c->stream;
+  LevelDev &f = c->L[0];
+  const int64_t n3 = 3 * f.nnod;
+  if (precision == M3_PREC_FP64) {
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.r0.p);
+    vcycle_impl<double>(c, [&](int l, auto fn) { fn(c->L[l].K.view()); }, vecs64);
+    hipLaunchKernelGGL(k_copy, dim3(nblk(n3)), dim3(256), 0, s, n3, f.d.p, d);
+  } else {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.r0f.p);
+    vcycle_impl<float>(
+        c,
+        [&](int l, auto fn) {
+          if (l == 0) fn(c->L[l].K.view32());
+          else fn(c->L[l].K.view16());
+        },
+        vecs32);
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(256), 0, s, n3, f.df.p, d);
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// FP32 / BF16 copies of the operators for the mixed-precision V-cycle
+void build_mixed(m3_ctx *c) {
+  hipStream_t s = c->stream;
+  for (int l = 0; l < c->nmg; ++l) {
+    LevelDev &lv = c->L[l];
+    const int64_t nv = 9 * lv.K.nslot;
+    const int64_t n3 = 3 * lv.nnod;
+    if (l < c->nmg - 1) {
+      if (l == 0) {
+        if (lv.K.val32.n != (size_t)nv) lv.K.val32.alloc(nv);
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(nv)), dim3(256), 0, s, nv, lv.K.val.p,
+                           lv.K.val32.p);
+      } else {
+        if (lv.K.val16.n != (size_t)nv) lv.K.val16.alloc(nv);
+        hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nv)), dim3(256), 0, s, nv, lv.K.val.p, lv.K.val16.p);
+      }
+    }
+    if (lv.dinv32.n != (size_t)n3) lv.dinv32.alloc(n3);
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, lv.dinv.p,
+                       lv.dinv32.p);
+    auto ensure = [&](DBuf<float> &b) { if (b.n != (size_t)n3) b.alloc(n3); };
+    ensure(lv.r0f); ensure(lv.r1f); ensure(lv.df); ensure(lv.dnf); ensure(lv.ddf); ensure(lv.rrf);
+    if (l < c->nmg - 1) {
+      const int64_t np = lv.P_val.n, nr = lv.R_val.n;
+      if (lv.P_val32.n != (size_t)np) lv.P_val32.alloc(np);
+      if (lv.R_val32.n != (size_t)nr) lv.R_val32.alloc(nr);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(np)), dim3(256), 0, s, np, lv.P_val.p, lv.P_val32.p);
+      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(nr)), dim3(256), 0, s, nr, lv.R_val.p, lv.R_val32.p);
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  c->have_mixed = true;
+}
+
+// ---- dot products and vector helpers ----------------------------------------------------
+static int red_blocks(int64_t n) {
+  int64_t b = (n + RB - 1) / RB;
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+double dot(m3_ctx *c, const double *a, const double *b, int64_t n) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RB), 0, c->stream, n, a, b, c->partial.p);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, c->stream, c->partial.p, nb, c->scal.p + 8);
+  HIPCHECK(hipMemcpyAsync(c->h_scal + 8, c->scal.p + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return c->h_scal[8];
+}
+
+double dot_diff(m3_ctx *c, const double *a, const double *cvec, const double *b, int64_t n) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(RB), 0, c->stream, n, a, cvec, b, c->partial.p, nb);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, c->stream, c->partial.p, nb, c->scal.p + 8);
+  HIPCHECK(hipMemcpyAsync(c->h_scal + 8, c->scal.p + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return c->h_scal[8];
+}
+
+double mean(m3_ctx *c, const double *a, int64_t n) {
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_vsum, dim3(nb), dim3(RB), 0, c->stream, n, a, c->partial.p);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, c->stream, c->partial.p, nb, c->scal.p + 8);
+  HIPCHECK(hipMemcpyAsync(c->h_scal + 8, c->scal.p + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return c->h_scal[8] / (double)n;
+}
+
+void axpy(hipStream_t s, double a, const double *x, double *y, int64_t n) {
+  hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(256), 0, s, n, a, x, y);
+}
+void xpby(hipStream_t s, const double *x, double b, double *y, int64_t n) {
+  hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(256), 0, s, n, x, b, y);
+}
+void copy(hipStream_t s, const double *x, double *y, int64_t n) {
+  hipLaunchKernelGGL(k_copy, dim3(nblk(n)), dim3(256), 0, s, n, x, y);
+}
+void scale_mask(hipStream_t s, const double *x, const double *m, double *y, int64_t n) {
+  hipLaunchKernelGGL(k_scale_mask, dim3(nblk(n)), dim3(256), 0, s, n, x, m, y);
+}
+void add_scalar(hipStream_t s, double *a, double v, int64_t n) { add_scalar_impl(s, a, v, n); }
+void scal(hipStream_t s, double a, double *y, int64_t n) {
+  hipLaunchKernelGGL(k_xpby, dim3(nblk(n)), dim3(256), 0, s, n, (const double *)nullptr, a, y);
+}
+
+// ---- one MGPCG iteration sequence, device-resident scalars -----------------------------
+// Returns the number of iterations; rrms history appended to hist.
+int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, int itmax,
+              int precision, std::vector<double> *hist) {
+  hipStream_t s = c->stream;
+  LevelDev &f = c->L[0];
+  const int64_t n3 = 3 * f.nnod;
+  const double nfree = (double)f.nfree;
+  double *r = c->cg_r.p, *rl = c->cg_rlast.p, *d = c->cg_d.p, *q = c->cg_q.p, *Kq = c->cg_Kq.p;
+  const int nb = red_blocks(n3);
+  // r = b - K x  (masked: fixed entries of b are zeroed)
+  hipLaunchKernelGGL(k_scale_mask, dim3(nblk(n3)), dim3(256), 0, s, n3, b, f.mask.p, c->cg_b.p);
+  hipLaunchKernelGGL((k_smooth<double, double, 2>), dim3(nblk(f.nnod)), dim3(256), 0, s, f.K.view(), f.nnod,
+                     x, c->cg_b.p, (const double *)nullptr, 0.0, (double *)nullptr, r, (double *)nullptr,
+                     (double *)nullptr);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RB), 0, s, n3, r, r, c->partial.p);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p + 5);
+  HIPCHECK(hipMemcpyAsync(c->h_scal + 5, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  double rrms = std::sqrt(c->h_scal[5] / nfree);
+  if (std::isnan(rrms)) throw M3Error(M3_E_NUMERIC, "RHS vector contains NaNs");
+  if (!is_abs) tol = tol * rrms;
+  if (hist) hist->push_back(rrms);
+  int it = 0;
+  const unsigned gK = nblk(f.nnod);
+  for (it = 1; it <= itmax; ++it) {
+    vcycle(c, r, d, precision);
+    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(RB), 0, s, n3, r, it == 1 ? (const double *)nullptr : rl, d,
+                       c->partial.p, nb);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p, it == 1 ? 1 : 0);
+    hipLaunchKernelGGL(k_cg_dir, dim3(nblk(n3)), dim3(256), 0, s, n3, d, q, c->scal.p);
+    hipLaunchKernelGGL((k_smooth<double, double, 9>), dim3(gK), dim3(256), 0, s, f.K.view(), f.nnod, q,
+                       (const double *)nullptr, (const double *)nullptr, 0.0, Kq, (double *)nullptr,
+                       (double *)nullptr, c->partial.p);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(RB), 0, s, c->partial.p, (int)gK, c->scal.p);
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(RB), 0, s, n3, x, q, r, rl, Kq, c->scal.p, c->partial.p);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p + 5);
+    HIPCHECK(hipMemcpyAsync(c->h_scal + 5, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    rrms = std::sqrt(c->h_scal[5] / nfree);
+    if (hist) hist->push_back(rrms);
+    if (std::isnan(rrms)) throw M3Error(M3_E_NUMERIC, "NaN in CG residual");
+    if (rrms < tol) break;
+  }
+  HIPCHECK(hipGetLastError());
+  return it > itmax ? itmax : it;
+}
+
+}  // namespace m3

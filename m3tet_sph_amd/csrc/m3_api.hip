@@ -1,0 +1,745 @@
+// C ABI of libm3stokes: mesh/BC set-up (host), assembly, Galerkin
+// restriction, coarse factorisation, MGPCG and the Patera pressure iteration
+// (flow3d_patera_mg_sph_p.m:374-681), all on one HIP stream.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+#include "m3_internal.h"
+
+namespace m3 {
+void chol_analyse(m3_ctx *c);
+}
+
+// ------------------------------------------------------------------ Sell build
+void Sell::build(int64_t nrow_, int64_t ncol_, const std::vector<int64_t> &ptr,
+                 const std::vector<int32_t> &cols, int bs_) {
+  nrow = nrow_;
+  ncol = ncol_;
+  bs = bs_;
+  const int64_t nsl = (nrow + 63) / 64;
+  h_slice_ptr.assign(nsl + 1, 0);
+  h_rowlen.assign(nrow, 0);
+  for (int64_t r = 0; r < nrow; ++r) h_rowlen[r] = (int32_t)(ptr[r + 1] - ptr[r]);
+  for (int64_t s = 0; s < nsl; ++s) {
+    int mx = 0;
+    for (int64_t r = 64 * s; r < std::min(nrow, 64 * s + 64); ++r) mx = std::max(mx, h_rowlen[r]);
+    h_slice_ptr[s + 1] = h_slice_ptr[s] + 64 * (int64_t)mx;
+  }
+  nslot = h_slice_ptr[nsl];
+  nnz = ptr[nrow];
+  h_col.assign(nslot, -1);
+  for (int64_t r = 0; r < nrow; ++r) {
+    const int64_t base = h_slice_ptr[r >> 6] + (r & 63);
+    for (int64_t k = ptr[r]; k < ptr[r + 1]; ++k) h_col[base + (k - ptr[r]) * 64] = cols[k];
+  }
+  slice_ptr.upload(h_slice_ptr);
+  rowlen.upload(h_rowlen);
+  col.upload(h_col);
+  val.alloc((size_t)nslot * bs);
+}
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+double secs(clk::time_point a, clk::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+
+// rows = nodes 0..nrow-1; for each element, row nodes rsel[] x col nodes csel[]
+void adjacency(const std::vector<int32_t> &el2nod, int64_t nel, int64_t nrow, int nr, const int *rsel,
+               int nc, const int *csel, std::vector<int64_t> &ptr, std::vector<int32_t> &cols) {
+  // node -> elements
+  std::vector<int64_t> eptr(nrow + 1, 0);
+  for (int64_t e = 0; e < nel; ++e)
+    for (int i = 0; i < nr; ++i) eptr[el2nod[10 * e + rsel[i]] + 1]++;
+  for (int64_t n = 0; n < nrow; ++n) eptr[n + 1] += eptr[n];
+  std::vector<int64_t> fill(eptr.begin(), eptr.end() - 1);
+  std::vector<int64_t> elist(eptr[nrow]);
+  for (int64_t e = 0; e < nel; ++e)
+    for (int i = 0; i < nr; ++i) elist[fill[el2nod[10 * e + rsel[i]]]++] = e;
+  std::vector<std::vector<int32_t>> rows(nrow);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t n = 0; n < nrow; ++n) {
+    std::vector<int32_t> &v = rows[n];
+    for (int64_t k = eptr[n]; k < eptr[n + 1]; ++k)
+      for (int j = 0; j < nc; ++j) v.push_back(el2nod[10 * elist[k] + csel[j]]);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  ptr.assign(nrow + 1, 0);
+  for (int64_t n = 0; n < nrow; ++n) ptr[n + 1] = ptr[n] + (int64_t)rows[n].size();
+  cols.resize(ptr[nrow]);
+  for (int64_t n = 0; n < nrow; ++n) std::copy(rows[n].begin(), rows[n].end(), cols.begin() + ptr[n]);
+}
+
+const int ALL10[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
+const int V4[4] = {0, 1, 2, 3};
+
+struct Timer {
+  hipEvent_t a, b;
+  hipStream_t s;
+  explicit Timer(hipStream_t st) : s(st) {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+  }
+  double stop() {
+    (void)hipEventRecord(b, s);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms * 1e-3;
+  }
+};
+
+int check_err(m3_ctx *c) {
+  int e = 0;
+  HIPCHECK(hipMemcpyAsync(&e, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return e;
+}
+
+void raise_flags(int e) {
+  if (e & 1) throw M3Error(M3_E_NUMERIC, "negative Jacobi");
+  if (e & 2) throw M3Error(M3_E_NUMERIC, "integration point has theta = 0 (det(J_CS) = Inf)");
+  if (e & 4) throw M3Error(M3_E_NUMERIC, "Calculated a zero or negative viscosity");
+  if (e & 8) throw M3Error(M3_E_MESH, "element block missing from the sparse pattern");
+  if (e & 16) throw M3Error(M3_E_NUMERIC, "coarse matrix is not positive definite");
+}
+
+template <class F>
+int guard(m3_ctx *c, F &&f) {
+  if (!c) return M3_E_ARG;
+  try {
+    HIPCHECK(hipSetDevice(c->device));
+    f();
+    c->err.clear();
+    return M3_OK;
+  } catch (const M3Error &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return M3_E_ARG;
+  }
+}
+
+void ensure(DBuf<double> &b, size_t n) {
+  if (b.n != n) b.alloc(n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int m3_create(m3_ctx **ctx, int hip_device) {
+  if (!ctx) return M3_E_ARG;
+  *ctx = nullptr;
+  m3_ctx *c = new m3_ctx();
+  c->device = hip_device;
+  try {
+    int ndev = 0;
+    HIPCHECK(hipGetDeviceCount(&ndev));
+    M3_REQUIRE(ndev > 0, M3_E_HIP, "no HIP device visible");
+    M3_REQUIRE(hip_device >= 0 && hip_device < ndev, M3_E_ARG, "invalid HIP device index");
+    HIPCHECK(hipSetDevice(hip_device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(hipHostMalloc(&c->h_scal, 64 * sizeof(double)));
+    c->scal.alloc(64);
+    c->errflag.alloc(1);
+    m3::upload_sf_tables();
+  } catch (const M3Error &e) {
+    c->err = e.what();
+    const int code = e.code;
+    *ctx = c;  // returned so that m3_last_error works; caller destroys it
+    return code;
+  }
+  *ctx = c;
+  return M3_OK;
+}
+
+void m3_destroy(m3_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  {
+    m3_ctx tmp;  // release device buffers before the stream
+    std::swap(tmp.L, c->L);
+  }
+  if (c->h_scal) (void)hipHostFree(c->h_scal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *m3_last_error(const m3_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
+  return guard(c, [&] {
+    M3_REQUIRE(m, M3_E_ARG, "null mesh descriptor");
+    M3_REQUIRE(m->nmg >= 2 && m->nmg <= M3_MAX_LEVELS, M3_E_ARG, "nmg must be in 2..8");
+    const int nmg = m->nmg;
+    c->nmg = nmg;
+    c->L.clear();
+    c->L.resize(nmg);
+    c->have_mesh = c->have_bc = c->assembled = c->restricted = c->factored = c->have_mixed = false;
+    for (int l = 0; l < nmg; ++l) {
+      LevelDev &lv = c->L[l];
+      lv.nel = m->nel[l];
+      lv.nnod = m->nnod[l];
+      lv.nV = m->nVnod[l];
+      M3_REQUIRE(m->EL2NOD[l] && lv.nel > 0 && lv.nnod > 0, M3_E_ARG, "empty level");
+      lv.h_el2nod.resize(10 * lv.nel);
+      for (int64_t i = 0; i < 10 * lv.nel; ++i) {
+        const int64_t v = (int64_t)m->EL2NOD[l][i] - 1;
+        M3_REQUIRE(v >= 0 && v < lv.nnod, M3_E_MESH, "EL2NOD entry out of range");
+        lv.h_el2nod[i] = (int32_t)v;
+      }
+      for (int64_t e = 0; e < lv.nel; ++e)
+        for (int i = 0; i < 4; ++i)
+          M3_REQUIRE(lv.h_el2nod[10 * e + i] < lv.nV, M3_E_MESH, "vertices must be numbered first");
+      if (l > 0) {
+        M3_REQUIRE(c->L[l - 1].nel == 8 * lv.nel, M3_E_MESH, "finer level must have 8x the elements");
+        M3_REQUIRE(c->L[l - 1].nV == lv.nnod, M3_E_MESH, "fine vertices must be the coarse P2 nodes");
+      }
+      lv.el2nod.upload(lv.h_el2nod);
+      std::vector<int64_t> ptr;
+      std::vector<int32_t> cols;
+      adjacency(lv.h_el2nod, lv.nel, lv.nnod, 10, ALL10, 10, ALL10, ptr, cols);
+      lv.K.build(lv.nnod, lv.nnod, ptr, cols, 9);
+      if (l < nmg - 1) lv.Kel.alloc((size_t)lv.nel * 465);
+    }
+    LevelDev &f = c->L[0];
+    c->nel = f.nel;
+    c->nnod = f.nnod;
+    c->nV = f.nV;
+    // G (velocity node x vertex), GT, M
+    {
+      std::vector<int64_t> ptr;
+      std::vector<int32_t> cols;
+      adjacency(f.h_el2nod, f.nel, f.nnod, 10, ALL10, 4, V4, ptr, cols);
+      c->G.build(f.nnod, f.nV, ptr, cols, 3);
+      adjacency(f.h_el2nod, f.nel, f.nV, 4, V4, 10, ALL10, ptr, cols);
+      c->GT.build(f.nV, f.nnod, ptr, cols, 3);
+      adjacency(f.h_el2nod, f.nel, f.nV, 4, V4, 4, V4, ptr, cols);
+      c->M.build(f.nV, f.nV, ptr, cols, 1);
+    }
+    // coordinates and nodal frames (make_rotation_matrix.m:69-110)
+    std::vector<double> X(3 * f.nnod), S(3 * f.nnod), fr(9 * f.nnod);
+    for (int64_t n = 0; n < f.nnod; ++n) {
+      for (int k = 0; k < 3; ++k) {
+        X[3 * n + k] = m->GCOORD[3 * n + k];
+        S[3 * n + k] = m->GCOORD_SPH[3 * n + k];
+      }
+      const double x = X[3 * n], y = X[3 * n + 1], z = X[3 * n + 2];
+      const double th = std::atan2(std::sqrt(x * x + y * y), z);
+      double ph = std::atan2(y, x);
+      if (ph < 0) ph += 2 * M_PI;
+      const double ct = std::cos(th), st = std::sin(th), cp = std::cos(ph), sp = std::sin(ph);
+      const double R[9] = {cp * ct, -sp, cp * st, sp * ct, cp, sp * st, -st, 0.0, ct};
+      for (int k = 0; k < 9; ++k) fr[9 * n + k] = R[k];
+    }
+    c->h_coord = X;
+    c->X.upload(X);
+    c->S.upload(S);
+    c->frames.upload(fr);
+    // classes
+    std::vector<uint8_t> cls(f.nel, 255);
+    std::vector<int32_t> iso(f.nel, -1);
+    for (int k = 0; k < 4; ++k) {
+      M3_REQUIRE(m->nels_class[k] == 0 || m->els_class[k], M3_E_ARG, "missing class list");
+      for (int64_t i = 0; i < m->nels_class[k]; ++i) {
+        const int64_t e = (int64_t)m->els_class[k][i] - 1;
+        M3_REQUIRE(e >= 0 && e < f.nel && cls[e] == 255, M3_E_MESH, "element classes must partition the mesh");
+        cls[e] = (uint8_t)k;
+        if (k == 3) iso[e] = (int32_t)i;
+      }
+    }
+    for (int64_t e = 0; e < f.nel; ++e) M3_REQUIRE(cls[e] != 255, M3_E_MESH, "element without class");
+    c->cls.upload(cls);
+    c->iso_idx.upload(iso);
+    c->n_iso = m->nels_class[3];
+    std::vector<double> X20(std::max<int64_t>(1, 60 * c->n_iso), 0.0);
+    for (int64_t i = 0; i < c->n_iso; ++i)
+      for (int n = 0; n < 20; ++n) {
+        const int64_t g = (int64_t)m->EL2NOD_cubic[20 * i + n] - 1;
+        M3_REQUIRE(g >= 0 && g < f.nnod + m->ncubic, M3_E_MESH, "EL2NOD_cubic out of range");
+        for (int k = 0; k < 3; ++k)
+          X20[60 * i + 3 * n + k] = g < f.nnod ? m->GCOORD[3 * g + k] : m->GCOORD_cubic[3 * (g - f.nnod) + k];
+      }
+    c->X20.upload(X20);
+    // transfers and child interpolation tables
+    for (int l = 0; l < nmg - 1; ++l) {
+      LevelDev &fl = c->L[l];
+      LevelDev &cl = c->L[l + 1];
+      M3_REQUIRE(m->Ic2f_ptr[l] && m->Ic2f_col[l] && m->Ic2f_val[l], M3_E_ARG, "missing Ic2f");
+      const int64_t *pp = m->Ic2f_ptr[l];
+      std::vector<int64_t> P_ptr(pp, pp + fl.nnod + 1);
+      const int64_t nnzP = P_ptr[fl.nnod];
+      std::vector<int32_t> P_col(m->Ic2f_col[l], m->Ic2f_col[l] + nnzP);
+      std::vector<double> P_val(m->Ic2f_val[l], m->Ic2f_val[l] + nnzP);
+      for (int64_t k = 0; k < nnzP; ++k)
+        M3_REQUIRE(P_col[k] >= 0 && P_col[k] < cl.nnod, M3_E_MESH, "Ic2f column out of range");
+      // transpose
+      std::vector<int64_t> R_ptr(cl.nnod + 1, 0);
+      for (int64_t k = 0; k < nnzP; ++k) R_ptr[P_col[k] + 1]++;
+      for (int64_t j = 0; j < cl.nnod; ++j) R_ptr[j + 1] += R_ptr[j];
+      std::vector<int64_t> fill(R_ptr.begin(), R_ptr.end() - 1);
+      std::vector<int32_t> R_col(nnzP);
+      std::vector<double> R_val(nnzP);
+      for (int64_t i = 0; i < fl.nnod; ++i)
+        for (int64_t k = P_ptr[i]; k < P_ptr[i + 1]; ++k) {
+          const int64_t q = fill[P_col[k]]++;
+          R_col[q] = (int32_t)i;
+          R_val[q] = P_val[k];
+        }
+      fl.P_ptr.upload(P_ptr);
+      fl.P_col.upload(P_col);
+      fl.P_val.upload(P_val);
+      fl.R_ptr.upload(R_ptr);
+      fl.R_col.upload(R_col);
+      fl.R_val.upload(R_val);
+      // Q_e[i][a] = Ic2f(node_e(i), node_parent(a)); every nonzero of the
+      // rows of a child must lie on its parent's nodes (nesting invariant)
+      std::map<std::vector<double>, int32_t> types;
+      std::vector<int32_t> ctype(fl.nel);
+      std::vector<double> qtab;
+      for (int64_t e = 0; e < fl.nel; ++e) {
+        const int64_t pc = e / 8;
+        std::vector<double> Q(100, 0.0);
+        for (int i = 0; i < 10; ++i) {
+          const int node = fl.h_el2nod[10 * e + i];
+          for (int64_t k = P_ptr[node]; k < P_ptr[node + 1]; ++k) {
+            int a = -1;
+            for (int q = 0; q < 10; ++q)
+              if (cl.h_el2nod[10 * pc + q] == P_col[k]) a = q;
+            M3_REQUIRE(a >= 0, M3_E_MESH,
+                       "Ic2f row of a fine node references a node outside the parent element");
+            Q[10 * i + a] = P_val[k];
+          }
+        }
+        auto it = types.find(Q);
+        if (it == types.end()) {
+          it = types.emplace(Q, (int32_t)types.size()).first;
+          qtab.insert(qtab.end(), Q.begin(), Q.end());
+        }
+        ctype[e] = it->second;
+      }
+      fl.child_type.upload(ctype);
+      fl.Qtab.upload(qtab);
+      fl.nqtypes = (int)types.size();
+    }
+    // work space
+    int64_t maxpart = 2048;
+    for (auto &lv : c->L) maxpart = std::max<int64_t>(maxpart, (lv.nnod + 255) / 256 + 16);
+    c->partial.alloc(2 * maxpart);
+    for (auto &lv : c->L) {
+      const size_t n3 = 3 * lv.nnod;
+      lv.r0.alloc(n3); lv.r1.alloc(n3); lv.d.alloc(n3); lv.dn.alloc(n3); lv.dd.alloc(n3); lv.rr.alloc(n3);
+      lv.dinv.alloc(n3);
+      lv.mask.alloc(n3);
+    }
+    const size_t n3 = 3 * f.nnod;
+    for (auto *b : {&c->cg_r, &c->cg_rlast, &c->cg_d, &c->cg_q, &c->cg_Kq, &c->cg_x, &c->cg_b, &c->Uv,
+                    &c->Zv, &c->Yv, &c->Rhs, &c->Fb, &c->bc_rhs, &c->U0})
+      b->alloc(n3);
+    for (auto *b : {&c->pr, &c->prlast, &c->pd, &c->pq, &c->pSq, &c->pP, &c->pMd, &c->LiMM}) b->alloc(f.nV);
+    c->visc.alloc(f.nnod);
+    c->dens.alloc(f.nnod);
+    c->have_mesh = true;
+  });
+}
+
+int m3_set_bc(m3_ctx *c, const int64_t *ifix, const double *vfix, int64_t nfix) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->have_mesh, M3_E_STATE, "m3_set_mesh must be called first");
+    M3_REQUIRE(nfix == 0 || (ifix && vfix), M3_E_ARG, "null BC arrays");
+    LevelDev &f = c->L[0];
+    const int64_t n3 = 3 * f.nnod;
+    std::vector<double> mask(n3, 1.0), U0(n3, 0.0);
+    for (int64_t i = 0; i < nfix; ++i) {
+      const int64_t d = ifix[i] - 1;
+      M3_REQUIRE(d >= 0 && d < n3, M3_E_ARG, "UBC.ifix out of range");
+      mask[d] = 0.0;
+      U0[d] = vfix[i];
+    }
+    c->h_ifix.assign(ifix, ifix + nfix);
+    c->h_vfix.assign(vfix, vfix + nfix);
+    for (int l = 0; l < c->nmg; ++l) {
+      LevelDev &lv = c->L[l];
+      std::vector<double> ml(mask.begin(), mask.begin() + 3 * lv.nnod);
+      lv.mask.upload(ml);
+      lv.nfree = (int64_t)std::count(ml.begin(), ml.end(), 1.0);
+    }
+    c->U0.upload(U0);
+    m3::chol_analyse(c);
+    c->have_bc = true;
+    c->assembled = c->restricted = c->factored = false;
+  });
+}
+
+static void do_assemble(m3_ctx *c, const double *Visc, const double *Dens, const m3_phys *phys) {
+  M3_REQUIRE(c->have_bc, M3_E_STATE, "m3_set_bc must be called before assembly");
+  M3_REQUIRE(phys && phys->nip == 14, M3_E_ARG, "only OPTS.nip = 14 is supported");
+  M3_REQUIRE(phys->type_Dmat == 0 || phys->type_Dmat == 1, M3_E_ARG, "type_Dmat must be 0 or 1");
+  c->phys = *phys;
+  LevelDev &f = c->L[0];
+  hipStream_t s = c->stream;
+  HIPCHECK(hipMemcpyAsync(c->visc.p, Visc, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(c->dens.p, Dens, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+  f.K.val.zero(s);
+  c->G.val.zero(s);
+  c->GT.val.zero(s);
+  c->M.val.zero(s);
+  c->Fb.zero(s);
+  c->bc_rhs.zero(s);
+  c->errflag.zero(s);
+  m3::launch_assemble(c, nullptr, nullptr, nullptr, nullptr, true);
+  m3::launch_fix_diag(c, 0);
+  m3::launch_lumped_mm(c);
+  raise_flags(check_err(c));
+  c->assembled = true;
+  c->restricted = c->factored = c->have_mixed = false;
+}
+
+int m3_assemble(m3_ctx *c, const double *Visc, const double *Dens, const m3_phys *phys) {
+  return guard(c, [&] { do_assemble(c, Visc, Dens, phys); });
+}
+
+int m3_element_matrices(m3_ctx *c, const double *Visc, const double *Dens, const m3_phys *phys,
+                        double *KKv, double *GGv, double *MMv, double *Fbv) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->have_mesh, M3_E_STATE, "m3_set_mesh must be called first");
+    M3_REQUIRE(phys && phys->nip == 14, M3_E_ARG, "only OPTS.nip = 14 is supported");
+    c->phys = *phys;
+    LevelDev &f = c->L[0];
+    hipStream_t s = c->stream;
+    HIPCHECK(hipMemcpyAsync(c->visc.p, Visc, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->dens.p, Dens, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+    DBuf<double> k, g, mm, fb;
+    k.alloc((size_t)465 * f.nel);
+    g.alloc((size_t)120 * f.nel);
+    mm.alloc((size_t)10 * f.nel);
+    fb.alloc((size_t)30 * f.nel);
+    if (!c->have_bc) {  // mask / U0 are read even when nothing is scattered
+      std::vector<double> one(3 * f.nnod, 1.0);
+      f.mask.upload(one);
+      c->U0.zero(s);
+    }
+    c->errflag.zero(s);
+    m3::launch_assemble(c, k.p, g.p, mm.p, fb.p, false);
+    raise_flags(check_err(c));
+    k.download(KKv, k.n);
+    g.download(GGv, g.n);
+    mm.download(MMv, mm.n);
+    fb.download(Fbv, fb.n);
+  });
+}
+
+static void do_restrict(m3_ctx *c) {
+  M3_REQUIRE(c->assembled, M3_E_STATE, "m3_assemble must be called first");
+  hipStream_t s = c->stream;
+  c->errflag.zero(s);
+  for (int l = 0; l < c->nmg - 1; ++l) {
+    c->L[l + 1].K.val.zero(s);
+    m3::launch_rap(c, l);
+    m3::launch_fix_diag(c, l + 1);
+  }
+  raise_flags(check_err(c));
+  c->restricted = true;
+  c->factored = c->have_mixed = false;
+}
+
+int m3_restrict_stiffmat(m3_ctx *c) {
+  return guard(c, [&] { do_restrict(c); });
+}
+
+static void do_factor(m3_ctx *c) {
+  M3_REQUIRE(c->restricted, M3_E_STATE, "m3_restrict_stiffmat must be called first");
+  c->errflag.zero(c->stream);
+  m3::chol_factor(c);
+  raise_flags(check_err(c));
+}
+
+int m3_cholesky_factorization(m3_ctx *c) {
+  return guard(c, [&] { do_factor(c); });
+}
+
+int m3_cholesky_solve(m3_ctx *c, const double *rhs, double *x) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->factored, M3_E_STATE, "m3_cholesky_factorization must be called first");
+    LevelDev &cl = c->L[c->nmg - 1];
+    const size_t n3 = 3 * cl.nnod;
+    HIPCHECK(hipMemcpyAsync(cl.rr.p, rhs, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+    m3::scale_mask(c->stream, cl.rr.p, cl.mask.p, cl.rr.p, (int64_t)n3);
+    m3::chol_solve_dev(c, cl.rr.p, cl.dn.p);
+    HIPCHECK(hipMemcpyAsync(x, cl.dn.p, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int m3_mgpcg(m3_ctx *c, const double *b, double *x, double tol, int32_t is_abs_tol, int32_t itmax,
+             int32_t precision, int32_t *it, double *rrms_vec) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->factored, M3_E_STATE, "operators are not ready (assemble, restrict, factorise)");
+    M3_REQUIRE(itmax >= 1, M3_E_ARG, "itmax must be >= 1");
+    LevelDev &f = c->L[0];
+    const int64_t n3 = 3 * f.nnod;
+    if (precision == M3_PREC_MIXED && !c->have_mixed) m3::build_mixed(c);
+    HIPCHECK(hipMemcpyAsync(c->Rhs.p, b, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->Zv.p, x, sizeof(double) * n3, hipMemcpyHostToDevice, c->stream));
+    m3::scale_mask(c->stream, c->Zv.p, f.mask.p, c->Zv.p, n3);
+    std::vector<double> hist;
+    const int n = m3::mgpcg_run(c, c->Zv.p, c->Rhs.p, tol, is_abs_tol != 0, itmax, precision, &hist);
+    HIPCHECK(hipMemcpyAsync(x, c->Zv.p, sizeof(double) * n3, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (it) *it = n;
+    if (rrms_vec)
+      for (size_t i = 0; i < hist.size() && i < (size_t)itmax + 1; ++i) rrms_vec[i] = hist[i];
+  });
+}
+
+// ---------------------------------------------------------------- Patera
+static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, m3_info *info) {
+  M3_REQUIRE(c->factored, M3_E_STATE, "operators are not ready (assemble, restrict, factorise)");
+  M3_REQUIRE(o && o->pc_Pat == 3, M3_E_ARG, "only pc_Pat = 3 is supported");
+  hipStream_t s = c->stream;
+  LevelDev &f = c->L[0];
+  const int64_t n3 = 3 * f.nnod, nV = f.nV;
+  const int prec = o->precision;
+  if (prec == M3_PREC_MIXED && !c->have_mixed) m3::build_mixed(c);
+  const double rtol_Pat = o->rtol_Pat, rtol_restart = 0.8 * o->rtol_Pat, wMM = o->wght_MM;
+  double rtol_U = o->rtol_U;
+  const int itmax_U = o->itmax_U;
+  double t_vel = 0, t_pres = 0;
+  auto t_start = clk::now();
+  double *Uf = c->Uv.p, *Z = c->Zv.p, *Y = c->Yv.p, *Rhs = c->Rhs.p;
+  double *r = c->pr.p, *rl = c->prlast.p, *d = c->pd.p, *q = c->pq.p, *Sq = c->pSq.p, *Pd = c->pP.p,
+         *Md = c->pMd.p;
+  DBuf<double> fbbc, gU0;
+  fbbc.alloc(n3);
+  gU0.alloc(nV);
+  // fbbc = mask (Fb - bc_rhs); gU0 = GG' U0
+  m3::copy(s, c->Fb.p, fbbc.p, n3);
+  m3::axpy(s, -1.0, c->bc_rhs.p, fbbc.p, n3);
+  m3::scale_mask(s, fbbc.p, f.mask.p, fbbc.p, n3);
+  m3::spmv_GT(s, c->GT, c->U0.p, gU0.p);
+  HIPCHECK(hipMemcpyAsync(Pd, P, sizeof(double) * nV, hipMemcpyHostToDevice, s));
+  if (o->warm_U) {
+    HIPCHECK(hipMemcpyAsync(Uf, U, sizeof(double) * n3, hipMemcpyHostToDevice, s));
+    m3::scale_mask(s, Uf, f.mask.p, Uf, n3);
+  } else {
+    Uf = c->Uv.p;
+    c->Uv.zero(s);
+  }
+  auto rhs_of_P = [&]() {  // Rhs = mask (Fb + GG P - bc_rhs)
+    m3::spmv_G(s, c->G, Pd, Rhs, f.mask.p);
+    m3::axpy(s, 1.0, fbbc.p, Rhs, n3);
+  };
+  auto pres_res = [&](bool rm0) {  // r = -GG' U (U = Uf + U0), mean removed
+    m3::spmv_GT(s, c->GT, Uf, r);
+    m3::axpy(s, 1.0, gU0.p, r, nV);
+    m3::scal(s, -1.0, r, nV);
+    if (rm0) m3::add_scalar(s, r, -m3::mean(c, r, nV), nV);
+  };
+  auto normdf = [&](const double *v, int64_t n, double nf) { return std::sqrt(m3::dot(c, v, v, n) / nf); };
+  auto vel_solve = [&](double *x, const double *b, double tol, bool abs) {
+    auto t0 = clk::now();
+    const int n = m3::mgpcg_run(c, x, b, tol, abs, itmax_U, prec, nullptr);
+    t_vel += secs(t0, clk::now());
+    return n;
+  };
+  rhs_of_P();
+  int nitU = vel_solve(Uf, Rhs, rtol_U * rtol_restart, false);
+  pres_res(true);
+  double norm_rP = normdf(r, nV, (double)nV);
+  const double norm_rP_init = norm_rP;
+  const double tol_Pat = rtol_Pat * norm_rP;
+  nitU += vel_solve(Uf, Rhs, rtol_U * rtol_restart * norm_rP, true);
+  const double tol_U_abs = rtol_U * tol_Pat;
+  int nitZ = 0, itPat = 0;
+  bool restart = true, conv_problem = false;
+  double rd = 0, norm_rP0 = norm_rP, tolZ = 0;
+  for (itPat = 1; itPat <= o->itmax_Pat; ++itPat) {
+    auto tp = clk::now();
+    // d = r ./ LMM; d += w LiMM (r - MM d)   (precondition_P, pc_Pat = 3)
+    m3::scale_mask(s, r, c->LiMM.p, d, nV);
+    m3::spmv_M(s, c->M, d, Md);
+    m3::copy(s, r, Sq, nV);
+    m3::axpy(s, -1.0, Md, Sq, nV);
+    m3::scale_mask(s, Sq, c->LiMM.p, Sq, nV);
+    m3::axpy(s, wMM, Sq, d, nV);
+    if (restart) {
+      m3::copy(s, d, q, nV);
+      restart = false;
+      norm_rP0 = norm_rP;
+      tolZ = rtol_U * std::max(rtol_restart * norm_rP0, tol_Pat);
+    } else {
+      const double rdlast = rd;
+      const double beta = m3::dot_diff(c, r, rl, d, nV) / rdlast;
+      m3::xpby(s, d, beta, q, nV);
+    }
+    m3::add_scalar(s, q, -m3::mean(c, q, nV), nV);
+    rd = m3::dot(c, r, d, nV);
+    m3::spmv_G(s, c->G, q, Y, f.mask.p);
+    c->Zv.zero(s);
+    t_pres += secs(tp, clk::now());
+    nitZ += vel_solve(Z, Y, tolZ, true);
+    tp = clk::now();
+    m3::spmv_GT(s, c->GT, Z, Sq);
+    const double qSq = m3::dot(c, q, Sq, nV);
+    m3::copy(s, r, rl, nV);
+    const double alpha = rd / qSq;
+    m3::axpy(s, alpha, q, Pd, nV);
+    m3::axpy(s, alpha, Z, Uf, n3);
+    m3::axpy(s, -alpha, Sq, r, nV);
+    m3::add_scalar(s, r, -m3::mean(c, r, nV), nV);
+    const double norm_rP_last = norm_rP;
+    norm_rP = normdf(r, nV, (double)nV);
+    const double norm_rP_min = std::min(norm_rP_last, norm_rP);
+    t_pres += secs(tp, clk::now());
+    if (norm_rP < tol_Pat && itPat >= o->itmin_Pat) break;
+    if (norm_rP < rtol_restart * norm_rP0 || norm_rP > 10 * norm_rP0) {
+      rhs_of_P();
+      nitU += vel_solve(Uf, Rhs, tol_U_abs, true);
+      pres_res(false);
+      const double norm_rP_true = normdf(r, nV, (double)nV);
+      if (norm_rP_true > 1.1 * norm_rP) rtol_U = rtol_U * 0.75 * (norm_rP / norm_rP_true);
+      if (norm_rP > 10 * norm_rP0 || norm_rP > 1.2 * norm_rP_min) {
+        if (conv_problem) break;
+        rtol_U = 0.1 * rtol_U;
+        conv_problem = true;
+      }
+      restart = true;
+    }
+  }
+  if (itPat > o->itmax_Pat) itPat = o->itmax_Pat;
+  rhs_of_P();
+  nitU += vel_solve(Uf, Rhs, tol_U_abs, true);
+  // U = Uf + U0 (rotated frame), P
+  m3::copy(s, Uf, Y, n3);
+  m3::axpy(s, 1.0, c->U0.p, Y, n3);
+  HIPCHECK(hipMemcpyAsync(U, Y, sizeof(double) * n3, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(P, Pd, sizeof(double) * nV, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < nV; ++i)
+    if (std::isnan(P[i])) throw M3Error(M3_E_NUMERIC, "Solution contains NaNs");
+  for (int64_t i = 0; i < n3; ++i)
+    if (std::isnan(U[i])) throw M3Error(M3_E_NUMERIC, "Solution contains NaNs");
+  if (info) {
+    info->nitP = itPat;
+    info->nitU = nitU;
+    info->nitZ = nitZ;
+    info->norm_rP = norm_rP;
+    info->norm_rP0 = norm_rP_init;
+    info->t_pressure = t_pres;
+    info->t_velocity = t_vel;
+  }
+  (void)t_start;
+}
+
+int m3_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *opts, m3_info *info) {
+  return guard(c, [&] { do_patera(c, U, P, opts, info); });
+}
+
+int m3_flow3d_patera_sph(m3_ctx *c, const double *Visc, const double *Dens, const m3_phys *phys,
+                         double *U, double *P, const m3_patera_opts *opts, m3_info *info) {
+  return guard(c, [&] {
+    auto t0 = clk::now();
+    Timer ta(c->stream);
+    do_assemble(c, Visc, Dens, phys);
+    const double t_as = ta.stop();
+    Timer tr(c->stream);
+    do_restrict(c);
+    const double t_re = tr.stop();
+    Timer tf(c->stream);
+    do_factor(c);
+    const double t_fa = tf.stop();
+    do_patera(c, U, P, opts, info);
+    if (info) {
+      info->t_assemble = t_as;
+      info->t_restrict = t_re;
+      info->t_factor = t_fa;
+      info->t_total = secs(t0, clk::now());
+    }
+  });
+}
+
+int m3_apply(m3_ctx *c, int32_t which, int32_t level, const double *x, double *y) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->assembled, M3_E_STATE, "m3_assemble must be called first");
+    M3_REQUIRE(level >= 0 && level < c->nmg, M3_E_ARG, "bad level");
+    hipStream_t s = c->stream;
+    DBuf<double> dx, dy;
+    LevelDev &lv = c->L[level];
+    int64_t nx = 0, ny = 0;
+    switch (which) {
+      case 0: nx = ny = 3 * lv.nnod; break;
+      case 1: nx = c->nV; ny = 3 * c->nnod; break;
+      case 2: nx = 3 * c->nnod; ny = c->nV; break;
+      case 3: nx = ny = c->nV; break;
+      case 4: M3_REQUIRE(level < c->nmg - 1, M3_E_ARG, "bad level");
+        nx = 3 * c->L[level + 1].nnod; ny = 3 * lv.nnod; break;
+      case 5: M3_REQUIRE(level < c->nmg - 1, M3_E_ARG, "bad level");
+        nx = 3 * lv.nnod; ny = 3 * c->L[level + 1].nnod; break;
+      default: throw M3Error(M3_E_ARG, "bad operator id");
+    }
+    M3_REQUIRE(which != 0 || level == 0 || c->restricted, M3_E_STATE, "coarse operators not built");
+    dx.upload(x, nx);
+    dy.alloc(ny);
+    switch (which) {
+      case 0: m3::spmv_K(s, lv, dx.p, dy.p); break;
+      case 1: m3::spmv_G(s, c->G, dx.p, dy.p, nullptr); break;
+      case 2: m3::spmv_GT(s, c->GT, dx.p, dy.p); break;
+      case 3: m3::spmv_M(s, c->M, dx.p, dy.p); break;
+      case 4: m3::prolong(s, lv, dx.p, nullptr, dy.p); break;
+      case 5: m3::restrict_(s, lv, c->L[level + 1], dx.p, dy.p); break;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+    dy.download(y, ny);
+  });
+}
+
+int m3_get_vector(m3_ctx *c, int32_t which, int32_t level, double *out) {
+  return guard(c, [&] {
+    M3_REQUIRE(level >= 0 && level < c->nmg, M3_E_ARG, "bad level");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    LevelDev &lv = c->L[level];
+    switch (which) {
+      case 0: c->Fb.download(out, 3 * c->nnod); break;
+      case 1: c->bc_rhs.download(out, 3 * c->nnod); break;
+      case 2: lv.dinv.download(out, 3 * lv.nnod);
+        for (int64_t i = 0; i < 3 * lv.nnod; ++i) out[i] = 1.0 / out[i];
+        break;
+      case 3: lv.mask.download(out, 3 * lv.nnod); break;
+      case 4: c->LiMM.download(out, c->nV);
+        for (int64_t i = 0; i < c->nV; ++i) out[i] = 1.0 / out[i];
+        break;
+      default: throw M3Error(M3_E_ARG, "bad vector id");
+    }
+  });
+}
+
+int m3_stats(m3_ctx *c, int64_t *out, int32_t nout) {
+  return guard(c, [&] {
+    std::vector<int64_t> v;
+    v.push_back(c->nmg);
+    for (int l = 0; l < c->nmg; ++l) v.push_back(c->L[l].K.nnz);
+    v.push_back(c->chol.n);
+    v.push_back(c->chol.T);
+    v.push_back(c->chol.ntile);
+    v.push_back(c->chol.nlev);
+    v.push_back(c->chol.nblev);
+    for (int l = 0; l < c->nmg; ++l) v.push_back(c->L[l].K.nslot);
+    for (int i = 0; i < nout && i < (int)v.size(); ++i) out[i] = v[i];
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,231 @@
+// Internal declarations of libm3stokes: device buffers, sliced block-sparse
+// storage, per-level state, kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/m3stokes.h"
+
+struct M3Error : std::runtime_error {
+  int code;
+  M3Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                  \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw M3Error(M3_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define M3_REQUIRE(cond, code, msg)                                                  \
+  do {                                                                               \
+    if (!(cond)) throw M3Error((code), (msg));                                       \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T *h, size_t count) {
+    if (count != n) alloc(count);
+    if (count) HIPCHECK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void upload(const std::vector<T> &h) { upload(h.data(), h.size()); }
+  void download(T *h, size_t count) const {
+    if (count) HIPCHECK(hipMemcpy(h, p, count * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  std::vector<T> to_host() const {
+    std::vector<T> h(n);
+    download(h.data(), n);
+    return h;
+  }
+  void zero(hipStream_t s) {
+    if (n) HIPCHECK(hipMemsetAsync(p, 0, n * sizeof(T), s));
+  }
+};
+
+// Sliced (SELL-64) block-sparse storage.  Rows are grouped in slices of 64 (one
+// wavefront); inside a slice the k-th stored block of each row sits at slot
+// slice_ptr[s] + 64*k + (row % 64), so the 64 lanes of a wave touch 64
+// consecutive slots.  Block values are stored structure-of-arrays,
+// val[v*nslot + slot] for v < bs, so every value load of a wave is one
+// contiguous 512-byte (FP64) segment.  Columns inside a row are sorted.
+template <class T>
+struct SellT {
+  const int64_t *slice_ptr;
+  const int32_t *rowlen;
+  const int32_t *col;
+  T *val;
+  int64_t nslot;
+};
+using SellView = SellT<double>;
+
+struct Sell {
+  int64_t nrow = 0, ncol = 0, nslot = 0, nnz = 0;
+  int bs = 9;  // values per stored block
+  std::vector<int64_t> h_slice_ptr;
+  std::vector<int32_t> h_rowlen, h_col;
+  DBuf<int64_t> slice_ptr;
+  DBuf<int32_t> rowlen, col;
+  DBuf<double> val;
+  DBuf<float> val32;
+  DBuf<uint16_t> val16;  // bf16 bit patterns
+  // build from sorted per-row column lists (CSR); uploads pattern, allocates val
+  void build(int64_t nrow_, int64_t ncol_, const std::vector<int64_t> &ptr,
+             const std::vector<int32_t> &cols, int bs_);
+  SellView view() { return SellView{slice_ptr.p, rowlen.p, col.p, val.p, nslot}; }
+  SellT<float> view32() { return SellT<float>{slice_ptr.p, rowlen.p, col.p, val32.p, nslot}; }
+  SellT<uint16_t> view16() { return SellT<uint16_t>{slice_ptr.p, rowlen.p, col.p, val16.p, nslot}; }
+};
+
+// Coarse tile Cholesky (nested-dissection order, 64x64 dense tiles).
+struct CholTiles {
+  int64_t n = 0;                 // free coarse dofs
+  int T = 0;                     // tile rows/cols
+  int64_t ntile = 0;             // stored lower tiles (incl. diagonal)
+  std::vector<int32_t> h_colptr, h_rowtile;   // tile CSC (row tiles sorted)
+  DBuf<int32_t> pos2dof;         // T*64: coarse dof (0-based) or -1 (padding)
+  DBuf<double> tiles;            // ntile*4096, column-major tiles
+  DBuf<double> tilesT;           // transposed copies (backward solve)
+  DBuf<double> dinv;             // T*4096 inverse of diagonal tiles of L
+  DBuf<double> dinvT;            // T*4096 transposes of dinv
+  DBuf<int64_t> gsrc, gdst;      // gather: BSR value index -> tile value index
+  DBuf<int32_t> diagtile;        // T: tile index of (J,J)
+  // factorisation schedule
+  int nlev = 0;
+  std::vector<int32_t> h_lev_ptr;          // nlev+1 into column list
+  DBuf<int32_t> lev_cols;                  // columns ordered by level
+  std::vector<int32_t> h_upd_ptr;          // nlev+1 into update tasks
+  DBuf<int32_t> upd_tile;                  // target tile index per task
+  DBuf<int64_t> upd_pair_ptr;              // task -> pair range
+  DBuf<int32_t> upd_pairs;                 // (t_IK, t_JK) pairs
+  std::vector<int32_t> h_trsm_ptr;         // nlev+1 into trsm tasks
+  DBuf<int32_t> trsm_tile, trsm_col;       // off-diagonal tile and its column J
+  // solves
+  std::vector<int32_t> h_flev_ptr, h_blev_ptr;
+  DBuf<int32_t> flev_cols, blev_cols;
+  DBuf<int32_t> frow_ptr, frow_tile, frow_col;  // row J: tiles (J,K) K<J and K
+  DBuf<int32_t> bcol_ptr, bcol_tile, bcol_row;  // col J: tiles (I,J) I>J and I
+  DBuf<double> work;             // T*64 solve vector
+  int nflev = 0, nblev = 0;
+};
+
+struct LevelDev {
+  int64_t nel = 0, nnod = 0, nV = 0;
+  std::vector<int32_t> h_el2nod;   // 0-based nel*10
+  DBuf<int32_t> el2nod;
+  Sell K;                          // 3x3 blocks
+  DBuf<double> mask;               // 3*nnod, 1 free / 0 fixed
+  DBuf<double> dinv;               // 3*nnod inverse diagonal (1 at fixed dofs)
+  DBuf<float> dinv32;
+  DBuf<double> Kel;                // nel*465 rotated + masked element matrices
+  // transfer to this level from level+1 (fine rows) and its transpose
+  DBuf<int64_t> P_ptr, R_ptr;
+  DBuf<int32_t> P_col, R_col;
+  DBuf<double> P_val, R_val;
+  DBuf<float> P_val32, R_val32;
+  DBuf<int32_t> child_type;        // per element: index into Qtab of level+1 transfer
+  DBuf<double> Qtab;               // ntypes*100 (10 fine local x 10 coarse local)
+  int nqtypes = 0;
+  int64_t nfree = 0;
+  // V-cycle work vectors (FP64 and FP32 variants)
+  DBuf<double> r0, r1, d, dn, dd, rr;
+  DBuf<float> r0f, r1f, df, dnf, ddf, rrf;
+};
+
+struct m3_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nmg = 0;
+  std::vector<LevelDev> L;
+  // finest-level geometry and operators
+  int64_t nel = 0, nnod = 0, nV = 0, n_iso = 0;
+  DBuf<uint8_t> cls;
+  DBuf<int32_t> iso_idx;
+  DBuf<double> X, S, X20, frames;
+  DBuf<double> visc, dens;
+  Sell G, GT, M;                   // GG (3x1), GG' (1x3), MM (scalar)
+  DBuf<double> Fb, bc_rhs, U0, LiMM;
+  DBuf<int32_t> errflag;
+  std::vector<int64_t> h_ifix;
+  std::vector<double> h_vfix;
+  bool have_mesh = false, have_bc = false, assembled = false, restricted = false,
+       factored = false, have_mixed = false;
+  CholTiles chol;
+  std::vector<double> h_coord;     // finest GCOORD row per node (for ordering)
+  // reductions
+  DBuf<double> partial;            // per-block partial sums
+  DBuf<double> scal;               // device scalars
+  double *h_scal = nullptr;        // pinned host mirror
+  // CG / Patera vectors (3*nnod or nV)
+  DBuf<double> cg_r, cg_rlast, cg_d, cg_q, cg_Kq, cg_x, cg_b;
+  DBuf<double> Uv, Zv, Yv, Rhs;
+  DBuf<double> pr, prlast, pd, pq, pSq, pP, pMd;
+  m3_phys phys{};
+};
+
+// ---------------------------------------------------------------- launchers
+namespace m3 {
+constexpr int NIP = 14;
+void upload_sf_tables();
+void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double *dbgF,
+                     bool write_global);
+void launch_rap(m3_ctx *c, int l);  // level l element matrices -> level l+1
+void launch_fix_diag(m3_ctx *c, int l);
+void launch_diag_inv(m3_ctx *c, int l);
+void launch_lumped_mm(m3_ctx *c);
+
+// sparse kernels
+void spmv_K(hipStream_t s, LevelDev &lv, const double *x, double *y);
+void spmv_G(hipStream_t s, Sell &G, const double *q, double *y, const double *mask);
+void spmv_GT(hipStream_t s, Sell &GT, const double *z, double *y);
+void spmv_M(hipStream_t s, Sell &M, const double *x, double *y);
+void prolong(hipStream_t s, LevelDev &fine, const double *dc, const double *dd, double *df);
+void restrict_(hipStream_t s, LevelDev &fine, LevelDev &coarse, const double *rf, double *rc);
+
+// V-cycle building blocks (templated on FP64 / FP32 vectors in k_sparse.hip)
+void vcycle(m3_ctx *c, const double *r, double *d, int precision);
+void chol_factor(m3_ctx *c);
+void chol_solve_dev(m3_ctx *c, const double *rhs, double *x);  // device vectors, coarse level
+
+// vector helpers
+double dot(m3_ctx *c, const double *a, const double *b, int64_t n);
+double dot_diff(m3_ctx *c, const double *a, const double *b, const double *d, int64_t n);
+void axpy(hipStream_t s, double a, const double *x, double *y, int64_t n);
+void xpby(hipStream_t s, const double *x, double b, double *y, int64_t n);  // y = x + b*y
+void copy(hipStream_t s, const double *x, double *y, int64_t n);
+void scale_mask(hipStream_t s, const double *x, const double *m, double *y, int64_t n);
+double mean(m3_ctx *c, const double *a, int64_t n);
+void add_scalar(hipStream_t s, double *a, double v, int64_t n);
+void scal(hipStream_t s, double a, double *y, int64_t n);
+void build_mixed(m3_ctx *c);
+int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, int itmax,
+              int precision, std::vector<double> *hist);
+}  // namespace m3
