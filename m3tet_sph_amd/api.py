@@ -296,7 +296,7 @@ class StokesContext:
         nmg = int(out[0])
         return dict(nmg=nmg, nnzb=[int(v) for v in out[1:1 + nmg]],
                     chol_n=int(out[1 + nmg]), chol_T=int(out[2 + nmg]), chol_tiles=int(out[3 + nmg]),
-                    chol_levels=int(out[4 + nmg]), chol_blevels=int(out[5 + nmg]),
+                    chol_levels=int(out[4 + nmg]), chol_heights=int(out[5 + nmg]),
                     nslot=[int(v) for v in out[6 + nmg:6 + 2 * nmg]])
 
 

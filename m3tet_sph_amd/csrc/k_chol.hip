@@ -3,38 +3,46 @@
 // lchol) and cholesky_solve.m:27-43 (x(perm) = L'\(L\b(perm))).
 //
 // Ordering: geometric nested dissection of the coarse nodes (recursive
-// coordinate bisection, separator = nodes of one half adjacent to the other),
-// all free dofs of a node kept together.  Symbolic factorisation runs at tile
-// granularity on the host (tile elimination tree), and the numeric phases run
-// as level-scheduled batches of tile tasks on the device:
+// coordinate bisection; separator = nodes of one half adjacent to the other),
+// all free dofs of a node kept together.  Every leaf and separator is a block
+// whose positions are padded to a multiple of 64, so tiles never straddle two
+// blocks and the tile elimination structure is the dissection tree.
+//
+// Numeric factorisation: level-scheduled batches of tile tasks
 //   update  C_IJ = A_IJ - sum_K L_IK L_JK'   (one workgroup per target tile)
-//   potrf   L_JJ = chol(C_JJ), and inv(L_JJ) (one workgroup per column)
+//   potrf   L_JJ = chol(C_JJ) and inv(L_JJ)  (one workgroup per column)
 //   trsm    L_IJ = C_IJ inv(L_JJ)'           (one workgroup per tile)
-// The triangular solves run one level-batch per launch; the diagonal block
-// solve is a product with the stored inv(L_JJ).
+// followed by the explicit inverse X_S = inv(L_SS) of each block's diagonal
+// part.  The triangular solves then take four launches per tree height:
+//   forward:  y_S = X_S w_S ;  w_A -= L_AS y_S       (ancestors A)
+//   backward: t_S = y_S - L_AS' x_A ;  x_S = X_S' t_S
+// each a batch of independent 64-row gathers.
 #include "m3_internal.h"
 
 #include <algorithm>
+#include <map>
 #include <numeric>
 
 namespace {
 
 constexpr int TS = 64;
 constexpr int TT = TS * TS;
+constexpr int LEAF = 40;  // nodes per leaf block
 
-// ---- host: nested dissection ----------------------------------------------------------
+// ---- host: nested dissection into tile-aligned blocks ----------------------------------
 struct NDState {
   const std::vector<int64_t> *ptr;
   const std::vector<int32_t> *adj;
   const std::vector<double> *xyz;
   std::vector<int32_t> stamp;
   int cur = 0;
-  std::vector<int32_t> order;
+  std::vector<std::vector<int32_t>> blocks;  // node lists in elimination order
 };
 
 void nd_rec(NDState &st, std::vector<int32_t> nodes) {
-  if (nodes.size() <= 40) {
-    st.order.insert(st.order.end(), nodes.begin(), nodes.end());
+  if (nodes.empty()) return;
+  if (nodes.size() <= (size_t)LEAF) {
+    st.blocks.push_back(nodes);
     return;
   }
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -52,29 +60,64 @@ void nd_rec(NDState &st, std::vector<int32_t> nodes) {
     return xa < xb || (xa == xb && a < b);
   });
   std::vector<int32_t> A(nodes.begin(), nodes.begin() + mid), B(nodes.begin() + mid, nodes.end());
+  const int sA = ++st.cur;
+  for (int n : A) st.stamp[n] = sA;
   const int sB = ++st.cur;
   for (int n : B) st.stamp[n] = sB;
-  std::vector<int32_t> A2, S;
+  // separator on the side with the smaller boundary
+  std::vector<int32_t> bA, bB;
   for (int n : A) {
-    bool touch = false;
-    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1] && !touch; ++k)
-      if (st.stamp[(*st.adj)[k]] == sB) touch = true;
-    (touch ? S : A2).push_back(n);
+    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
+      if (st.stamp[(*st.adj)[k]] == sB) { bA.push_back(n); break; }
   }
-  ++st.cur;  // invalidate stamps of B for nested calls
+  for (int n : B) {
+    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
+      if (st.stamp[(*st.adj)[k]] == sA) { bB.push_back(n); break; }
+  }
+  const bool sepA = bA.size() <= bB.size();
+  const std::vector<int32_t> &S = sepA ? bA : bB;
+  const int sS = ++st.cur;
+  for (int n : S) st.stamp[n] = sS;
+  std::vector<int32_t> A2, B2;
+  for (int n : A)
+    if (st.stamp[n] != sS) A2.push_back(n);
+  for (int n : B)
+    if (st.stamp[n] != sS) B2.push_back(n);
+  std::vector<int32_t> Sv = S;
   nd_rec(st, A2);
-  nd_rec(st, B);
-  st.order.insert(st.order.end(), S.begin(), S.end());
+  nd_rec(st, B2);
+  if (!Sv.empty()) st.blocks.push_back(Sv);
 }
 
-// ---- device kernels -------------------------------------------------------------------
+// ---- device kernels --------------------------------------------------------------------
 __global__ void k_gather(int64_t n, const int64_t *src, const int64_t *dst, const double *val,
-                         int64_t nslot, double *tiles) {
+                         double *tiles) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) tiles[dst[i]] = val[src[i]];
 }
 
-// C_IJ -= sum_pairs L_IK L_JK'   (tiles column-major)
+// acc(64x64, 4x4 per thread) += A * B' or A * B over a list of tile pairs
+__device__ inline void tile_mm(const double *A, const double *B, bool transB, double (*sA)[TS + 1],
+                               double (*sB)[TS + 1], double acc[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  __syncthreads();
+  for (int t = threadIdx.x; t < TT; t += 256) {
+    const int r = t & 63, c = t >> 6;  // column-major element (r,c)
+    sA[r][c] = A[t];
+    if (transB) sB[r][c] = B[t];       // B used as B'
+    else sB[c][r] = B[t];              // store B' so that the loop below is shared
+  }
+  __syncthreads();
+  for (int k = 0; k < TS; ++k) {
+    double a[4], b[4];
+    for (int i = 0; i < 4; ++i) a[i] = sA[4 * ty + i][k];
+    for (int j = 0; j < 4; ++j) b[j] = sB[4 * tx + j][k];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+  }
+}
+
+// C_IJ -= sum_pairs L_IK L_JK'
 __global__ __launch_bounds__(256) void k_update(const int32_t *task_tile, const int64_t *pair_ptr,
                                                 const int32_t *pairs, int task0, double *tiles) {
   const int task = task0 + blockIdx.x;
@@ -83,31 +126,12 @@ __global__ __launch_bounds__(256) void k_update(const int32_t *task_tile, const 
   double *C = tiles + (int64_t)task_tile[task] * TT;
   __shared__ double sA[TS][TS + 1];
   __shared__ double sB[TS][TS + 1];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // output rows 4*ty.., cols 4*tx..
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   double acc[4][4] = {};
-  for (int64_t p = p0; p < p1; ++p) {
-    const double *A = tiles + (int64_t)pairs[2 * p] * TT;      // L_IK
-    const double *B = tiles + (int64_t)pairs[2 * p + 1] * TT;  // L_JK
-    __syncthreads();
-    for (int t = threadIdx.x; t < TT; t += 256) {
-      const int r = t & 63, c = t >> 6;  // column-major element (r,c)
-      sA[r][c] = A[t];
-      sB[r][c] = B[t];
-    }
-    __syncthreads();
-    for (int k = 0; k < TS; ++k) {
-      double a[4], b[4];
-      for (int i = 0; i < 4; ++i) a[i] = sA[4 * ty + i][k];
-      for (int j = 0; j < 4; ++j) b[j] = sB[4 * tx + j][k];
-      for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
-    }
-  }
+  for (int64_t p = p0; p < p1; ++p)
+    tile_mm(tiles + (int64_t)pairs[2 * p] * TT, tiles + (int64_t)pairs[2 * p + 1] * TT, true, sA, sB, acc);
   for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      const int r = 4 * ty + i, c = 4 * tx + j;
-      C[r + TS * c] -= acc[i][j];
-    }
+    for (int j = 0; j < 4; ++j) C[(4 * ty + i) + TS * (4 * tx + j)] -= acc[i][j];
 }
 
 // Cholesky of the diagonal tile (in LDS) and inverse of the factor.
@@ -117,7 +141,7 @@ __global__ __launch_bounds__(256) void k_potrf(const int32_t *cols, int col0, co
   const int J = cols[col0 + blockIdx.x];
   double *A = tiles + (int64_t)diagtile[J] * TT;
   __shared__ double L[TS][TS + 1];
-  __shared__ double X[TS][TS + 1];
+  __shared__ double Xs[TS][TS + 1];
   for (int t = threadIdx.x; t < TT; t += 256) {
     const int r = t & 63, c = t >> 6;
     double v = A[t];
@@ -142,13 +166,12 @@ __global__ __launch_bounds__(256) void k_potrf(const int32_t *cols, int col0, co
     }
     __syncthreads();
   }
-  // X = inv(L): column j by forward substitution (thread j < 64)
-  if (threadIdx.x < TS) {
+  if (threadIdx.x < TS) {  // X = inv(L), column j by forward substitution
     const int j = threadIdx.x;
     for (int i = 0; i < TS; ++i) {
       double s = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; ++k) s -= L[i][k] * X[k][j];
-      X[i][j] = (i >= j) ? s / L[i][i] : 0.0;
+      for (int k = j; k < i; ++k) s -= L[i][k] * Xs[k][j];
+      Xs[i][j] = (i >= j) ? s / L[i][i] : 0.0;
     }
   }
   __syncthreads();
@@ -157,8 +180,8 @@ __global__ __launch_bounds__(256) void k_potrf(const int32_t *cols, int col0, co
   for (int t = threadIdx.x; t < TT; t += 256) {
     const int r = t & 63, c = t >> 6;
     A[t] = (r >= c) ? L[r][c] : 0.0;
-    Di[t] = X[r][c];
-    DiT[t] = X[c][r];
+    Di[t] = Xs[r][c];
+    DiT[t] = Xs[c][r];
   }
 }
 
@@ -167,24 +190,12 @@ __global__ __launch_bounds__(256) void k_trsm(const int32_t *tile, const int32_t
                                               double *tiles, double *tilesT, const double *dinv) {
   const int task = task0 + blockIdx.x;
   double *C = tiles + (int64_t)tile[task] * TT;
-  const double *D = dinv + (int64_t)col[task] * TT;
-  __shared__ double sC[TS][TS + 1];
-  __shared__ double sD[TS][TS + 1];
-  for (int t = threadIdx.x; t < TT; t += 256) {
-    const int r = t & 63, c = t >> 6;
-    sC[r][c] = C[t];
-    sD[r][c] = D[t];
-  }
+  __shared__ double sA[TS][TS + 1];
+  __shared__ double sB[TS][TS + 1];
+  double acc[4][4] = {};
+  tile_mm(C, dinv + (int64_t)col[task] * TT, true, sA, sB, acc);
   __syncthreads();
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4] = {};
-  for (int k = 0; k < TS; ++k) {
-    double a[4], b[4];
-    for (int i = 0; i < 4; ++i) a[i] = sC[4 * ty + i][k];
-    for (int j = 0; j < 4; ++j) b[j] = sD[4 * tx + j][k];
-    for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
-  }
   double *CT = tilesT + (int64_t)tile[task] * TT;
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
@@ -194,58 +205,73 @@ __global__ __launch_bounds__(256) void k_trsm(const int32_t *tile, const int32_t
     }
 }
 
-// forward: y_J = inv(L_JJ) (b_J - sum_K L_JK y_K)
-__global__ __launch_bounds__(64) void k_fwd(const int32_t *cols, int col0, const int32_t *rptr,
-                                            const int32_t *rtile, const int32_t *rcol,
-                                            const double *tiles, const double *dinv, double *w) {
-  const int J = cols[col0 + blockIdx.x];
-  const int i = threadIdx.x;
-  __shared__ double sy[TS];
-  double acc = w[(int64_t)J * TS + i];
-  for (int k = rptr[J]; k < rptr[J + 1]; ++k) {
-    const double *Lt = tiles + (int64_t)rtile[k] * TT;
-    const double *y = w + (int64_t)rcol[k] * TS;
-    __syncthreads();
-    sy[i] = y[i];
-    __syncthreads();
-    double s = 0;
-    for (int c = 0; c < TS; ++c) s += Lt[i + TS * c] * sy[c];
-    acc -= s;
+// X_ij = -inv(L_ii) sum_k L_ik X_kj  (i > j inside a block), X_jj = inv(L_jj)
+__global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t *inv_row,
+                                             const int64_t *pair_ptr, const int32_t *pairs, int task0,
+                                             const double *tiles, const double *dinv, double *X,
+                                             double *XT) {
+  const int task = task0 + blockIdx.x;
+  double *Xo = X + (int64_t)inv_x[task] * TT;
+  double *XTo = XT + (int64_t)inv_x[task] * TT;
+  const double *Di = dinv + (int64_t)inv_row[task] * TT;
+  const int64_t p0 = pair_ptr[task], p1 = pair_ptr[task + 1];
+  __shared__ double sA[TS][TS + 1];
+  __shared__ double sB[TS][TS + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  if (p0 == p1) {  // diagonal tile of the block
+    for (int t = threadIdx.x; t < TT; t += 256) {
+      const int r = t & 63, c = t >> 6;
+      Xo[t] = Di[t];
+      XTo[c + TS * r] = Di[t];
+    }
+    return;
   }
+  double acc[4][4] = {};
+  for (int64_t p = p0; p < p1; ++p)
+    tile_mm(tiles + (int64_t)pairs[2 * p] * TT, X + (int64_t)pairs[2 * p + 1] * TT, false, sA, sB, acc);
+  // S = acc (64x64) -> LDS, then Xo = -Di * S
   __syncthreads();
-  sy[i] = acc;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) sB[4 * tx + j][4 * ty + i] = acc[i][j];  // sB[c][r] = S[r][c]
+  for (int t = threadIdx.x; t < TT; t += 256) sA[t & 63][t >> 6] = Di[t];
   __syncthreads();
-  const double *D = dinv + (int64_t)J * TT;
-  double s = 0;
-  for (int c = 0; c < TS; ++c) s += D[i + TS * c] * sy[c];
-  w[(int64_t)J * TS + i] = s;
+  double o[4][4] = {};
+  for (int k = 0; k < TS; ++k) {
+    double a[4], b[4];
+    for (int i = 0; i < 4; ++i) a[i] = sA[4 * ty + i][k];
+    for (int j = 0; j < 4; ++j) b[j] = sB[4 * tx + j][k];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) o[i][j] += a[i] * b[j];
+  }
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * ty + i, c = 4 * tx + j;
+      Xo[r + TS * c] = -o[i][j];
+      XTo[c + TS * r] = -o[i][j];
+    }
 }
 
-// backward: x_J = inv(L_JJ)' (y_J - sum_I L_IJ' x_I)
-__global__ __launch_bounds__(64) void k_bwd(const int32_t *cols, int col0, const int32_t *cptr,
-                                            const int32_t *ctile, const int32_t *crow,
-                                            const double *tilesT, const double *dinvT, double *w) {
-  const int J = cols[col0 + blockIdx.x];
-  const int j = threadIdx.x;
-  __shared__ double sx[TS];
-  double acc = w[(int64_t)J * TS + j];
-  for (int k = cptr[J]; k < cptr[J + 1]; ++k) {
-    const double *LT = tilesT + (int64_t)ctile[k] * TT;  // L_IJ' column-major
-    const double *x = w + (int64_t)crow[k] * TS;
-    __syncthreads();
-    sx[j] = x[j];
-    __syncthreads();
-    double s = 0;
-    for (int i = 0; i < TS; ++i) s += LT[j + TS * i] * sx[i];
-    acc -= s;
+// out[row] = base[row] + sign * sum_p M_p v[vcol_p]   (64-row gathers)
+__global__ __launch_bounds__(256) void k_gtask(int task0, const int32_t *row, const int64_t *tptr,
+                                               const int32_t *mat, const int32_t *vcol,
+                                               const double *mats, const double *v, const double *base,
+                                               double sign, double *out) {
+  const int task = task0 + blockIdx.x;
+  const int r = row[task];
+  const int i = threadIdx.x & 63, part = threadIdx.x >> 6;  // 4 waves split the columns
+  __shared__ double red[4][TS];
+  double acc = 0;
+  for (int64_t p = tptr[task]; p < tptr[task + 1]; ++p) {
+    const double *M = mats + (int64_t)mat[p] * TT;
+    const double *vv = v + (int64_t)vcol[p] * TS;
+    for (int c = part; c < TS; c += 4) acc += M[i + TS * c] * vv[c];
   }
+  red[part][i] = acc;
   __syncthreads();
-  sx[j] = acc;
-  __syncthreads();
-  const double *DT = dinvT + (int64_t)J * TT;
-  double s = 0;
-  for (int i = 0; i < TS; ++i) s += DT[j + TS * i] * sx[i];
-  w[(int64_t)J * TS + j] = s;
+  if (part == 0) {
+    const double s = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+    out[(int64_t)r * TS + i] = (base ? base[(int64_t)r * TS + i] : 0.0) + sign * s;
+  }
 }
 
 __global__ void k_to_tiles(int64_t npos, const int32_t *pos2dof, const double *x, double *w) {
@@ -262,20 +288,46 @@ __global__ void k_from_tiles(int64_t npos, const int32_t *pos2dof, const double 
   if (d >= 0) x[d] = w[i];
 }
 
+__global__ void k_zero(int64_t n, double *x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <class T>
+void up(DBuf<T> &b, const std::vector<T> &h) {
+  if (h.empty()) b.upload(std::vector<T>{T(0)});
+  else b.upload(h);
+}
+
+// a gather phase built per height
+struct PhaseBuilder {
+  std::vector<int32_t> row, mat, vcol, hptr{0};
+  std::vector<int64_t> tptr{0};
+  void task(int r) { row.push_back(r); }
+  void item(int m, int c) { mat.push_back(m); vcol.push_back(c); }
+  void end_task() { tptr.push_back((int64_t)mat.size()); }
+  void end_height() { hptr.push_back((int32_t)row.size()); }
+  void upload(CholTiles::Phase &p) {
+    p.h_ptr = hptr;
+    up(p.row, row);
+    up(p.mat, mat);
+    up(p.vcol, vcol);
+    up(p.tptr, tptr);
+  }
+};
 
 }  // namespace
 
 namespace m3 {
 
-// Symbolic analysis: ordering, tile structure, schedules.  Needs the coarsest
-// level pattern, its mask and the node coordinates.
+// Symbolic analysis: ordering, tile structure, schedules.
 void chol_analyse(m3_ctx *c) {
   LevelDev &cl = c->L[c->nmg - 1];
   CholTiles &ch = c->chol;
   const int64_t nn = cl.nnod;
   const std::vector<double> mask = cl.mask.to_host();
-  // node adjacency (CSR) from the sliced pattern
   std::vector<int64_t> ptr(nn + 1, 0);
   std::vector<int32_t> adj;
   const Sell &K = cl.K;
@@ -293,18 +345,33 @@ void chol_analyse(m3_ctx *c) {
   for (int64_t n = 0; n < nn; ++n)
     if (mask[3 * n] + mask[3 * n + 1] + mask[3 * n + 2] > 0) nodes.push_back((int32_t)n);
   nd_rec(st, nodes);
-  std::vector<int32_t> perm;  // position -> dof
-  for (int n : st.order)
-    for (int k = 0; k < 3; ++k)
-      if (mask[3 * n + k] != 0.0) perm.push_back(3 * n + k);
-  ch.n = (int64_t)perm.size();
-  ch.T = (int)((ch.n + TS - 1) / TS);
+  // positions, padded per block
+  std::vector<int32_t> pos2dof, blk_t0, blk_t1;
+  int64_t nfree = 0;
+  for (auto &b : st.blocks) {
+    std::vector<int32_t> dofs;
+    for (int n : b)
+      for (int k = 0; k < 3; ++k)
+        if (mask[3 * (int64_t)n + k] != 0.0) dofs.push_back(3 * n + k);
+    if (dofs.empty()) continue;
+    const int t0 = (int)(pos2dof.size() / TS);
+    pos2dof.insert(pos2dof.end(), dofs.begin(), dofs.end());
+    nfree += (int64_t)dofs.size();
+    while (pos2dof.size() % TS) pos2dof.push_back(-1);
+    blk_t0.push_back(t0);
+    blk_t1.push_back((int)(pos2dof.size() / TS));
+  }
+  ch.n = nfree;
+  ch.T = (int)(pos2dof.size() / TS);
+  ch.nblock = (int)blk_t0.size();
   const int T = ch.T;
+  std::vector<int32_t> blk_of(T);
+  for (int b = 0; b < ch.nblock; ++b)
+    for (int t = blk_t0[b]; t < blk_t1[b]; ++t) blk_of[t] = b;
   std::vector<int32_t> inv(3 * nn, -1);
-  for (size_t i = 0; i < perm.size(); ++i) inv[perm[i]] = (int32_t)i;
-  std::vector<int32_t> pos2dof((size_t)T * TS, -1);
-  for (size_t i = 0; i < perm.size(); ++i) pos2dof[i] = perm[i];
-  // tile structure of A (lower)
+  for (size_t i = 0; i < pos2dof.size(); ++i)
+    if (pos2dof[i] >= 0) inv[pos2dof[i]] = (int32_t)i;
+  // tile structure of A (lower); blocks are dense on their diagonal part
   std::vector<std::vector<int32_t>> Acol(T);
   for (int64_t n = 0; n < nn; ++n)
     for (int64_t k = ptr[n]; k < ptr[n + 1]; ++k) {
@@ -320,18 +387,17 @@ void chol_analyse(m3_ctx *c) {
       }
     }
   for (int J = 0; J < T; ++J) {
-    Acol[J].push_back(J);
+    for (int I = J; I < blk_t1[blk_of[J]]; ++I) Acol[J].push_back(I);
     std::sort(Acol[J].begin(), Acol[J].end());
     Acol[J].erase(std::unique(Acol[J].begin(), Acol[J].end()), Acol[J].end());
   }
   // tile-level symbolic factorisation
-  std::vector<std::vector<int32_t>> S(T);
-  std::vector<std::vector<int32_t>> children(T);
+  std::vector<std::vector<int32_t>> S(T), children(T);
   for (int J = 0; J < T; ++J) {
     std::vector<int32_t> s = Acol[J];
-    for (int K : children[J])
-      for (int I : S[K])
-        if (I > K && I >= J) s.push_back(I);
+    for (int Kc : children[J])
+      for (int I : S[Kc])
+        if (I > Kc && I >= J) s.push_back(I);
     std::sort(s.begin(), s.end());
     s.erase(std::unique(s.begin(), s.end()), s.end());
     S[J] = s;
@@ -347,13 +413,11 @@ void chol_analyse(m3_ctx *c) {
     const auto it = std::lower_bound(b, e, I);
     return (it != e && *it == I) ? (int32_t)(it - ch.h_rowtile.begin()) : -1;
   };
-  // row lists: for row I, (K, t_IK) with K < I
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> rows(T);
-  for (int K = 0; K < T; ++K)
-    for (int32_t t = ch.h_colptr[K] + 1; t < ch.h_colptr[K + 1]; ++t)
-      rows[ch.h_rowtile[t]].push_back({K, t});
-  // levels
-  std::vector<int> lev(T, 0), blev(T, 0);
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> rows(T);  // row I: (K, t_IK), K < I
+  for (int Kc = 0; Kc < T; ++Kc)
+    for (int32_t t = ch.h_colptr[Kc] + 1; t < ch.h_colptr[Kc + 1]; ++t) rows[ch.h_rowtile[t]].push_back({Kc, t});
+  // factorisation levels
+  std::vector<int> lev(T, 0);
   int nlev = 0;
   for (int J = 0; J < T; ++J) {
     int l = 0;
@@ -361,38 +425,17 @@ void chol_analyse(m3_ctx *c) {
     lev[J] = l;
     nlev = std::max(nlev, l + 1);
   }
-  int nblev = 0;
-  for (int J = T - 1; J >= 0; --J) {
-    int l = 0;
-    for (int32_t t = ch.h_colptr[J] + 1; t < ch.h_colptr[J + 1]; ++t) l = std::max(l, blev[ch.h_rowtile[t]] + 1);
-    blev[J] = l;
-    nblev = std::max(nblev, l + 1);
-  }
   ch.nlev = nlev;
-  ch.nflev = nlev;
-  ch.nblev = nblev;
-  std::vector<int32_t> lev_cols, blev_cols;
-  ch.h_lev_ptr.assign(nlev + 1, 0);
-  ch.h_blev_ptr.assign(nblev + 1, 0);
-  for (int l = 0; l < nlev; ++l) {
-    for (int J = 0; J < T; ++J)
-      if (lev[J] == l) lev_cols.push_back(J);
-    ch.h_lev_ptr[l + 1] = (int32_t)lev_cols.size();
-  }
-  for (int l = 0; l < nblev; ++l) {
-    for (int J = 0; J < T; ++J)
-      if (blev[J] == l) blev_cols.push_back(J);
-    ch.h_blev_ptr[l + 1] = (int32_t)blev_cols.size();
-  }
-  ch.h_flev_ptr = ch.h_lev_ptr;
-  // update and trsm tasks by level
-  std::vector<int32_t> upd_tile, trsm_tile, trsm_col, pairs;
+  std::vector<int32_t> lev_cols, upd_tile, trsm_tile, trsm_col, pairs;
   std::vector<int64_t> pair_ptr{0};
+  ch.h_lev_ptr.assign(nlev + 1, 0);
   ch.h_upd_ptr.assign(nlev + 1, 0);
   ch.h_trsm_ptr.assign(nlev + 1, 0);
+  std::vector<std::vector<int32_t>> cols_at(nlev);
+  for (int J = 0; J < T; ++J) cols_at[lev[J]].push_back(J);
   for (int l = 0; l < nlev; ++l) {
-    for (int32_t q = ch.h_lev_ptr[l]; q < ch.h_lev_ptr[l + 1]; ++q) {
-      const int J = lev_cols[q];
+    for (int J : cols_at[l]) {
+      lev_cols.push_back(J);
       for (int32_t t = ch.h_colptr[J]; t < ch.h_colptr[J + 1]; ++t) {
         const int I = ch.h_rowtile[t];
         upd_tile.push_back(t);
@@ -410,23 +453,109 @@ void chol_analyse(m3_ctx *c) {
         }
       }
     }
+    ch.h_lev_ptr[l + 1] = (int32_t)lev_cols.size();
     ch.h_upd_ptr[l + 1] = (int32_t)upd_tile.size();
     ch.h_trsm_ptr[l + 1] = (int32_t)trsm_tile.size();
   }
-  // solve lists
-  std::vector<int32_t> frow_ptr{0}, frow_tile, frow_col, bcol_ptr{0}, bcol_tile, bcol_row, diagt(T);
-  for (int J = 0; J < T; ++J) {
-    for (auto &kt : rows[J]) {
-      frow_tile.push_back(kt.second);
-      frow_col.push_back(kt.first);
+  // block heights from the structure of L: a block comes after every block
+  // that has tiles in its rows
+  std::vector<int> hb(ch.nblock, 0);
+  for (int b = 0; b < ch.nblock; ++b) {
+    int h = 0;
+    for (int I = blk_t0[b]; I < blk_t1[b]; ++I)
+      for (auto &kt : rows[I]) {
+        const int bk = blk_of[kt.first];
+        if (bk != b) h = std::max(h, hb[bk] + 1);
+      }
+    hb[b] = h;
+  }
+  int H = 0;
+  for (int b = 0; b < ch.nblock; ++b) H = std::max(H, hb[b] + 1);
+  ch.H = H;
+  // inverse tiles of the diagonal blocks: block-local m x m, X index
+  std::vector<int64_t> offX(ch.nblock + 1, 0);
+  int maxm = 0;
+  for (int b = 0; b < ch.nblock; ++b) {
+    const int m = blk_t1[b] - blk_t0[b];
+    maxm = std::max(maxm, m);
+    offX[b + 1] = offX[b] + (int64_t)m * m;
+  }
+  ch.maxm = maxm;
+  ch.nX = offX[ch.nblock];
+  auto xidx = [&](int b, int i, int j) {  // global tile rows/cols i >= j in block b
+    const int m = blk_t1[b] - blk_t0[b];
+    return (int32_t)(offX[b] + (i - blk_t0[b]) + (int64_t)m * (j - blk_t0[b]));
+  };
+  std::vector<int32_t> inv_x, inv_row, inv_pairs;
+  std::vector<int64_t> inv_pair_ptr{0};
+  ch.h_inv_ptr.assign(maxm + 1, 0);
+  for (int d = 0; d < maxm; ++d) {
+    for (int b = 0; b < ch.nblock; ++b) {
+      for (int j = blk_t0[b]; j + d < blk_t1[b]; ++j) {
+        const int i = j + d;
+        inv_x.push_back(xidx(b, i, j));
+        inv_row.push_back(i);
+        if (d > 0)
+          for (int k = j; k < i; ++k) {
+            const int32_t tl = tile_of(i, k);
+            if (tl >= 0) {
+              inv_pairs.push_back(tl);
+              inv_pairs.push_back(xidx(b, k, j));
+            }
+          }
+        inv_pair_ptr.push_back((int64_t)inv_pairs.size() / 2);
+      }
     }
-    frow_ptr.push_back((int32_t)frow_tile.size());
-    for (int32_t t = ch.h_colptr[J] + 1; t < ch.h_colptr[J + 1]; ++t) {
-      bcol_tile.push_back(t);
-      bcol_row.push_back(ch.h_rowtile[t]);
+    ch.h_inv_ptr[d + 1] = (int32_t)inv_x.size();
+  }
+  // solve phases
+  std::vector<std::vector<int>> blocks_at(H);
+  for (int b = 0; b < ch.nblock; ++b) blocks_at[hb[b]].push_back(b);
+  PhaseBuilder fd, fu, bu, bd;
+  for (int h = 0; h < H; ++h) {
+    // forward diagonal: y_i = sum_{j<=i in block} X_ij w_j
+    for (int b : blocks_at[h])
+      for (int i = blk_t0[b]; i < blk_t1[b]; ++i) {
+        fd.task(i);
+        for (int j = blk_t0[b]; j <= i; ++j) fd.item(xidx(b, i, j), j);
+        fd.end_task();
+      }
+    fd.end_height();
+    // forward update: w_I -= sum L_Ij y_j for j in blocks of height h
+    std::map<int, std::vector<std::pair<int32_t, int32_t>>> tgt;
+    for (int b : blocks_at[h])
+      for (int j = blk_t0[b]; j < blk_t1[b]; ++j)
+        for (int32_t t = ch.h_colptr[j]; t < ch.h_colptr[j + 1]; ++t) {
+          const int I = ch.h_rowtile[t];
+          if (I >= blk_t1[b]) tgt[I].push_back({t, j});
+        }
+    for (auto &kv : tgt) {
+      fu.task(kv.first);
+      for (auto &tj : kv.second) fu.item(tj.first, tj.second);
+      fu.end_task();
     }
-    bcol_ptr.push_back((int32_t)bcol_tile.size());
-    diagt[J] = ch.h_colptr[J];
+    fu.end_height();
+  }
+  for (int h = H - 1; h >= 0; --h) {
+    // backward update: t_j = y_j - sum_{I outside block} L_Ij' x_I
+    for (int b : blocks_at[h])
+      for (int j = blk_t0[b]; j < blk_t1[b]; ++j) {
+        bu.task(j);
+        for (int32_t t = ch.h_colptr[j]; t < ch.h_colptr[j + 1]; ++t) {
+          const int I = ch.h_rowtile[t];
+          if (I >= blk_t1[b]) bu.item(t, I);
+        }
+        bu.end_task();
+      }
+    bu.end_height();
+    // backward diagonal: x_j = sum_{i>=j in block} X_ij' t_i
+    for (int b : blocks_at[h])
+      for (int j = blk_t0[b]; j < blk_t1[b]; ++j) {
+        bd.task(j);
+        for (int i = j; i < blk_t1[b]; ++i) bd.item(xidx(b, i, j), i);
+        bd.end_task();
+      }
+    bd.end_height();
   }
   // gather map from the coarse operator values (SoA, 9*nslot)
   std::vector<int64_t> gsrc, gdst;
@@ -448,29 +577,33 @@ void chol_analyse(m3_ctx *c) {
       }
     }
   }
-  ch.pos2dof.upload(pos2dof);
-  ch.lev_cols.upload(lev_cols);
-  ch.blev_cols.upload(blev_cols);
-  ch.flev_cols.upload(lev_cols);
-  ch.upd_tile.upload(upd_tile);
-  ch.upd_pair_ptr.upload(pair_ptr);
-  ch.upd_pairs.upload(pairs.empty() ? std::vector<int32_t>{0, 0} : pairs);
-  ch.trsm_tile.upload(trsm_tile.empty() ? std::vector<int32_t>{0} : trsm_tile);
-  ch.trsm_col.upload(trsm_col.empty() ? std::vector<int32_t>{0} : trsm_col);
-  ch.frow_ptr.upload(frow_ptr);
-  ch.frow_tile.upload(frow_tile.empty() ? std::vector<int32_t>{0} : frow_tile);
-  ch.frow_col.upload(frow_col.empty() ? std::vector<int32_t>{0} : frow_col);
-  ch.bcol_ptr.upload(bcol_ptr);
-  ch.bcol_tile.upload(bcol_tile.empty() ? std::vector<int32_t>{0} : bcol_tile);
-  ch.bcol_row.upload(bcol_row.empty() ? std::vector<int32_t>{0} : bcol_row);
-  ch.diagtile.upload(diagt);
-  ch.gsrc.upload(gsrc);
-  ch.gdst.upload(gdst);
+  std::vector<int32_t> diagt(T);
+  for (int J = 0; J < T; ++J) diagt[J] = ch.h_colptr[J];
+  up(ch.pos2dof, pos2dof);
+  up(ch.lev_cols, lev_cols);
+  up(ch.upd_tile, upd_tile);
+  up(ch.upd_pair_ptr, pair_ptr);
+  up(ch.upd_pairs, pairs);
+  up(ch.trsm_tile, trsm_tile);
+  up(ch.trsm_col, trsm_col);
+  up(ch.inv_x, inv_x);
+  up(ch.inv_row, inv_row);
+  up(ch.inv_pair_ptr, inv_pair_ptr);
+  up(ch.inv_pairs, inv_pairs);
+  up(ch.diagtile, diagt);
+  up(ch.gsrc, gsrc);
+  up(ch.gdst, gdst);
+  fd.upload(ch.fdiag);
+  fu.upload(ch.fupd);
+  bu.upload(ch.bupd);
+  bd.upload(ch.bdiag);
   ch.tiles.alloc((size_t)ch.ntile * TT);
   ch.tilesT.alloc((size_t)ch.ntile * TT);
   ch.dinv.alloc((size_t)T * TT);
   ch.dinvT.alloc((size_t)T * TT);
-  ch.work.alloc((size_t)T * TS);
+  ch.X.alloc((size_t)ch.nX * TT);
+  ch.XT.alloc((size_t)ch.nX * TT);
+  for (auto *b : {&ch.w, &ch.y, &ch.t, &ch.x}) b->alloc((size_t)T * TS);
   c->factored = false;
 }
 
@@ -479,13 +612,9 @@ void chol_factor(m3_ctx *c) {
   LevelDev &cl = c->L[c->nmg - 1];
   hipStream_t s = c->stream;
   ch.tiles.zero(s);
-  // dinvT lives in the upper half of dinv's allocation pair: use tilesT for
-  // off-diagonal transposes and a second buffer for inverse transposes
-  static_assert(TS == 64, "tile size");
-  if (ch.work.n < (size_t)ch.T * TS) ch.work.alloc((size_t)ch.T * TS);
   const int64_t ng = (int64_t)ch.gsrc.n;
   hipLaunchKernelGGL(k_gather, dim3(nblk(ng)), dim3(256), 0, s, ng, ch.gsrc.p, ch.gdst.p, cl.K.val.p,
-                     cl.K.nslot, ch.tiles.p);
+                     ch.tiles.p);
   for (int l = 0; l < ch.nlev; ++l) {
     const int u0 = ch.h_upd_ptr[l], u1 = ch.h_upd_ptr[l + 1];
     if (u1 > u0)
@@ -499,8 +628,22 @@ void chol_factor(m3_ctx *c) {
       hipLaunchKernelGGL(k_trsm, dim3(t1 - t0), dim3(256), 0, s, ch.trsm_tile.p, ch.trsm_col.p, t0,
                          ch.tiles.p, ch.tilesT.p, ch.dinv.p);
   }
+  for (int d = 0; d < ch.maxm; ++d) {
+    const int a = ch.h_inv_ptr[d], b = ch.h_inv_ptr[d + 1];
+    if (b > a)
+      hipLaunchKernelGGL(k_inv, dim3(b - a), dim3(256), 0, s, ch.inv_x.p, ch.inv_row.p, ch.inv_pair_ptr.p,
+                         ch.inv_pairs.p, a, ch.tiles.p, ch.dinv.p, ch.X.p, ch.XT.p);
+  }
   HIPCHECK(hipGetLastError());
   c->factored = true;
+}
+
+static void run_phase(hipStream_t s, CholTiles::Phase &p, int h, const double *mats, const double *v,
+                      const double *base, double sign, double *out) {
+  const int a = p.h_ptr[h], b = p.h_ptr[h + 1];
+  if (b > a)
+    hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, a, p.row.p, p.tptr.p, p.mat.p, p.vcol.p, mats,
+                       v, base, sign, out);
 }
 
 void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
@@ -508,19 +651,18 @@ void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
   LevelDev &cl = c->L[c->nmg - 1];
   hipStream_t s = c->stream;
   const int64_t npos = (int64_t)ch.T * TS;
-  hipLaunchKernelGGL(k_to_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.work.p);
-  for (int l = 0; l < ch.nflev; ++l) {
-    const int c0 = ch.h_lev_ptr[l], c1 = ch.h_lev_ptr[l + 1];
-    hipLaunchKernelGGL(k_fwd, dim3(c1 - c0), dim3(64), 0, s, ch.flev_cols.p, c0, ch.frow_ptr.p,
-                       ch.frow_tile.p, ch.frow_col.p, ch.tiles.p, ch.dinv.p, ch.work.p);
+  hipLaunchKernelGGL(k_to_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.w.p);
+  for (int h = 0; h < ch.H; ++h) {
+    run_phase(s, ch.fdiag, h, ch.X.p, ch.w.p, nullptr, 1.0, ch.y.p);
+    run_phase(s, ch.fupd, h, ch.tiles.p, ch.y.p, ch.w.p, -1.0, ch.w.p);
   }
-  for (int l = 0; l < ch.nblev; ++l) {
-    const int c0 = ch.h_blev_ptr[l], c1 = ch.h_blev_ptr[l + 1];
-    hipLaunchKernelGGL(k_bwd, dim3(c1 - c0), dim3(64), 0, s, ch.blev_cols.p, c0, ch.bcol_ptr.p,
-                       ch.bcol_tile.p, ch.bcol_row.p, ch.tilesT.p, ch.dinvT.p, ch.work.p);
+  for (int k = 0; k < ch.H; ++k) {
+    run_phase(s, ch.bupd, k, ch.tilesT.p, ch.x.p, ch.y.p, -1.0, ch.t.p);
+    run_phase(s, ch.bdiag, k, ch.XT.p, ch.t.p, nullptr, 1.0, ch.x.p);
   }
-  HIPCHECK(hipMemsetAsync(x, 0, sizeof(double) * 3 * cl.nnod, s));
-  hipLaunchKernelGGL(k_from_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, ch.work.p, x);
+  const int64_t n3 = 3 * cl.nnod;
+  hipLaunchKernelGGL(k_zero, dim3(nblk(n3)), dim3(256), 0, s, n3, x);
+  hipLaunchKernelGGL(k_from_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, ch.x.p, x);
   HIPCHECK(hipGetLastError());
 }
 

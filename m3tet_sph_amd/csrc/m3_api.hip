@@ -736,7 +736,7 @@ int m3_stats(m3_ctx *c, int64_t *out, int32_t nout) {
     v.push_back(c->chol.T);
     v.push_back(c->chol.ntile);
     v.push_back(c->chol.nlev);
-    v.push_back(c->chol.nblev);
+    v.push_back(c->chol.H);
     for (int l = 0; l < c->nmg; ++l) v.push_back(c->L[l].K.nslot);
     for (int i = 0; i < nout && i < (int)v.size(); ++i) out[i] = v[i];
   });

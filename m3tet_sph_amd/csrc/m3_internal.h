@@ -104,36 +104,38 @@ struct Sell {
   SellT<uint16_t> view16() { return SellT<uint16_t>{slice_ptr.p, rowlen.p, col.p, val16.p, nslot}; }
 };
 
-// Coarse tile Cholesky (nested-dissection order, 64x64 dense tiles).
+// Coarse sparse Cholesky in 64x64 dense tiles, nested-dissection blocks
+// aligned to tiles (one block = one leaf or separator = one supernode).
 struct CholTiles {
   int64_t n = 0;                 // free coarse dofs
-  int T = 0;                     // tile rows/cols
-  int64_t ntile = 0;             // stored lower tiles (incl. diagonal)
+  int T = 0;                     // tile rows/cols (including padding)
+  int64_t ntile = 0;             // stored lower tiles of L (incl. diagonal)
   std::vector<int32_t> h_colptr, h_rowtile;   // tile CSC (row tiles sorted)
   DBuf<int32_t> pos2dof;         // T*64: coarse dof (0-based) or -1 (padding)
-  DBuf<double> tiles;            // ntile*4096, column-major tiles
-  DBuf<double> tilesT;           // transposed copies (backward solve)
-  DBuf<double> dinv;             // T*4096 inverse of diagonal tiles of L
-  DBuf<double> dinvT;            // T*4096 transposes of dinv
-  DBuf<int64_t> gsrc, gdst;      // gather: BSR value index -> tile value index
+  DBuf<double> tiles, tilesT;    // L tiles (column-major) and transposed copies
+  DBuf<double> dinv, dinvT;      // inverses of the diagonal tiles of L
+  DBuf<int64_t> gsrc, gdst;      // gather: operator value index -> tile value index
   DBuf<int32_t> diagtile;        // T: tile index of (J,J)
-  // factorisation schedule
+  // factorisation: tile tasks by dependency level
   int nlev = 0;
-  std::vector<int32_t> h_lev_ptr;          // nlev+1 into column list
-  DBuf<int32_t> lev_cols;                  // columns ordered by level
-  std::vector<int32_t> h_upd_ptr;          // nlev+1 into update tasks
-  DBuf<int32_t> upd_tile;                  // target tile index per task
-  DBuf<int64_t> upd_pair_ptr;              // task -> pair range
-  DBuf<int32_t> upd_pairs;                 // (t_IK, t_JK) pairs
-  std::vector<int32_t> h_trsm_ptr;         // nlev+1 into trsm tasks
-  DBuf<int32_t> trsm_tile, trsm_col;       // off-diagonal tile and its column J
-  // solves
-  std::vector<int32_t> h_flev_ptr, h_blev_ptr;
-  DBuf<int32_t> flev_cols, blev_cols;
-  DBuf<int32_t> frow_ptr, frow_tile, frow_col;  // row J: tiles (J,K) K<J and K
-  DBuf<int32_t> bcol_ptr, bcol_tile, bcol_row;  // col J: tiles (I,J) I>J and I
-  DBuf<double> work;             // T*64 solve vector
-  int nflev = 0, nblev = 0;
+  std::vector<int32_t> h_lev_ptr, h_upd_ptr, h_trsm_ptr;
+  DBuf<int32_t> lev_cols, upd_tile, upd_pairs, trsm_tile, trsm_col;
+  DBuf<int64_t> upd_pair_ptr;
+  // supernodes: explicit inverse of each block's diagonal part
+  int nblock = 0, H = 0, maxm = 0;
+  int64_t nX = 0;
+  DBuf<double> X, XT;            // inverse tiles (block-local m x m, column-major tiles)
+  std::vector<int32_t> h_inv_ptr;            // per step d: task range
+  DBuf<int32_t> inv_x, inv_row;              // target X tile, row tile (for dinv)
+  DBuf<int64_t> inv_pair_ptr;
+  DBuf<int32_t> inv_pairs;                   // (L tile, X tile) pairs
+  // solves: four gather-task phases per height
+  struct Phase {
+    std::vector<int32_t> h_ptr;  // per height: task range
+    DBuf<int32_t> row, mat, vcol;
+    DBuf<int64_t> tptr;
+  } fdiag, fupd, bupd, bdiag;
+  DBuf<double> w, y, t, x;       // T*64 solve vectors
 };
 
 struct LevelDev {
