@@ -113,6 +113,11 @@ typedef struct {
   int32_t precision;   /* M3_PREC_FP64 or M3_PREC_MIXED */
   int32_t warm_U;      /* 0: U starts from the BC values as in the reference
                           (flow3d_patera_mg_sph_p.m:230); 1: use U as given */
+  double rtol_U_final; /* 0: reference behaviour.  > 0: the final velocity solve
+                          (flow3d_patera_mg_sph_p.m:630-632) also stops only once
+                          |r_U| <= rtol_U_final |f_U| (fixed-denominator residual),
+                          with up to itmax_U_final iterations */
+  int32_t itmax_U_final;
 } m3_patera_opts;
 
 typedef struct {
@@ -174,6 +179,26 @@ int m3_get_vector(m3_ctx *ctx, int32_t which, int32_t level, double *out);
 /* Sizes: n[0]=levels, n[1..] per-level block nonzeros of K, then coarse factor
  * statistics (free dofs, tiles, stored tiles, levels). */
 int m3_stats(m3_ctx *ctx, int64_t *out, int32_t nout);
+
+/* Residuals of (U, P) for the assembled problem, RMS norms (normdf.m:22):
+ * out[0] = |r_U| with r_U = Fb + GG P - bc_rhs - K1 U on the free dofs,
+ * out[1] = |f_U| with f_U = Fb + GG P - bc_rhs on the free dofs,
+ * out[2] = |r_P| with r_P = -GG' U (mean removed).  No reference counterpart:
+ * it evaluates the fixed-denominator residual used to stop and compare solves. */
+int m3_residual(m3_ctx *ctx, const double *U, const double *P, double *out);
+
+/* Kernel timing with HIP events on the context stream.  enable=1 starts
+ * recording every fine-level smoother sweep (and fine SpMV), enable=0 stops.
+ * m3_profile_read: out[0] launches, out[1] total seconds of the FP64 sweeps,
+ * out[2] launches, out[3] seconds of the mixed-precision sweeps.  No reference
+ * counterpart (measurement only). */
+int m3_profile(m3_ctx *ctx, int32_t enable);
+int m3_profile_read(m3_ctx *ctx, double *out, int32_t nout);
+
+/* Level operator as block CSR (3x3 blocks, row-major inside a block):
+ * rowptr (nnod+1), col (nnzb), val (9*nnzb).  Pass NULL pointers to query
+ * nothing but sizes via m3_stats.  No reference counterpart (diagnostics). */
+int m3_get_operator(m3_ctx *ctx, int32_t level, int64_t *rowptr, int32_t *col, double *val);
 
 #ifdef __cplusplus
 }

@@ -65,7 +65,8 @@ class Phys(C.Structure):
 class PateraOpts(C.Structure):
     _fields_ = [("rtol_Pat", C.c_double), ("rtol_U", C.c_double), ("itmax_U", C.c_int32),
                 ("itmin_Pat", C.c_int32), ("itmax_Pat", C.c_int32), ("pc_Pat", C.c_int32),
-                ("wght_MM", C.c_double), ("precision", C.c_int32), ("warm_U", C.c_int32)]
+                ("wght_MM", C.c_double), ("precision", C.c_int32), ("warm_U", C.c_int32),
+                ("rtol_U_final", C.c_double), ("itmax_U_final", C.c_int32)]
 
 
 class Info(C.Structure):
@@ -121,13 +122,15 @@ def phys_struct(phys, nip=14, type_Dmat="23rd"):
 
 
 def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=120,
-                wght_MM=1.0, precision="fp64", warm_U=False):
-    """Defaults of flow3d_patera_mg_sph_p.m:70-98."""
+                wght_MM=1.0, precision="fp64", warm_U=False, rtol_U_final=0.0, itmax_U_final=60):
+    """Defaults of flow3d_patera_mg_sph_p.m:70-98 (rtol_U_final=0 keeps the
+    reference's final velocity tolerance)."""
     o = PateraOpts()
     o.rtol_Pat, o.rtol_U, o.itmax_U = rtol_Pat, rtol_U, itmax_U
     o.itmin_Pat, o.itmax_Pat, o.pc_Pat, o.wght_MM = itmin_Pat, itmax_Pat, 3, wght_MM
     o.precision = PREC[precision]
     o.warm_U = int(warm_U)
+    o.rtol_U_final, o.itmax_U_final = rtol_U_final, itmax_U_final
     return o
 
 
@@ -374,3 +377,43 @@ def flow3d_patera_sph(VAR, MESH, COMM, UBC, SETTINGS, PHYSICS, NUMSCALE, dt=None
     VAR.update(Uth=U[0::3].copy(), Uph=U[1::3].copy(), Ur=U[2::3].copy(), Ux=Uxyz[:, 0].copy(),
                Uy=Uxyz[:, 1].copy(), Uz=Uxyz[:, 2].copy(), P=P)
     return VAR, info
+
+
+# ---- measurement helpers (no reference counterpart) ------------------------
+LIB.m3_residual.argtypes = [C.c_void_p, D, D, D]
+LIB.m3_profile.argtypes = [C.c_void_p, C.c_int32]
+LIB.m3_profile_read.argtypes = [C.c_void_p, D, C.c_int32]
+LIB.m3_get_operator.argtypes = [C.c_void_p, C.c_int32, I64, I32, D]
+for _f in ("m3_residual", "m3_profile", "m3_profile_read", "m3_get_operator"):
+    getattr(LIB, _f).restype = C.c_int
+
+
+def residual(ctx, U, P):
+    """(|r_U|, |f_U|, |r_P|) RMS norms for the assembled problem."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    out = np.zeros(3)
+    ctx._check(LIB.m3_residual(ctx._h, _p(U), _p(P), _p(out)))
+    return out
+
+
+def profile(ctx, enable):
+    ctx._check(LIB.m3_profile(ctx._h, int(enable)))
+
+
+def profile_read(ctx):
+    out = np.zeros(4)
+    ctx._check(LIB.m3_profile_read(ctx._h, _p(out), 4))
+    return dict(n_fp64=int(out[0]), t_fp64=out[1], n_mixed=int(out[2]), t_mixed=out[3])
+
+
+def get_operator(ctx, level):
+    """Level operator as scipy BSR (3x3 blocks)."""
+    st = ctx.stats()
+    nn = ctx.mesh.levels[level].nnod
+    nb = st["nnzb"][level]
+    rp = np.zeros(nn + 1, dtype=np.int64)
+    col = np.zeros(nb, dtype=np.int32)
+    val = np.zeros(9 * nb)
+    ctx._check(LIB.m3_get_operator(ctx._h, level, _p(rp, I64), _p(col, I32), _p(val)))
+    return sp.bsr_matrix((val.reshape(nb, 3, 3), col, rp), shape=(3 * nn, 3 * nn))

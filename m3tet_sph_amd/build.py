@@ -18,7 +18,7 @@ def build(verbose=True, jobs=4):
     procs = []
     common = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-              "-fopenmp", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+              "-munsafe-fp-atomics", "-fopenmp", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
               "-Wno-unused-but-set-variable", "-Wno-unknown-pragmas"]
     for src in SOURCES:
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))

@@ -117,13 +117,15 @@ __device__ inline void tile_mm(const double *A, const double *B, bool transB, do
   }
 }
 
-// C_IJ -= sum_pairs L_IK L_JK'
-__global__ __launch_bounds__(256) void k_update(const int32_t *task_tile, const int64_t *pair_ptr,
-                                                const int32_t *pairs, int task0, double *tiles) {
-  const int task = task0 + blockIdx.x;
-  const int64_t p0 = pair_ptr[task], p1 = pair_ptr[task + 1];
-  if (p0 == p1) return;
-  double *C = tiles + (int64_t)task_tile[task] * TT;
+// C_IJ -= sum_pairs L_IK L_JK'  (one workgroup per chunk of at most UCH pairs
+// of a target tile; chunks of one tile subtract atomically)
+constexpr int UCH = 8;
+__global__ __launch_bounds__(256) void k_update(const int32_t *chunk_tile, const int64_t *pbeg,
+                                                const int64_t *pend, const int32_t *pairs, int chunk0,
+                                                double *tiles) {
+  const int ck = chunk0 + blockIdx.x;
+  const int64_t p0 = pbeg[ck], p1 = pend[ck];
+  double *C = tiles + (int64_t)chunk_tile[ck] * TT;
   __shared__ double sA[TS][TS + 1];
   __shared__ double sB[TS][TS + 1];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void k_update(const int32_t *task_tile, const 
   for (int64_t p = p0; p < p1; ++p)
     tile_mm(tiles + (int64_t)pairs[2 * p] * TT, tiles + (int64_t)pairs[2 * p + 1] * TT, true, sA, sB, acc);
   for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) C[(4 * ty + i) + TS * (4 * tx + j)] -= acc[i][j];
+    for (int j = 0; j < 4; ++j) atomicAdd(C + (4 * ty + i) + TS * (4 * tx + j), -acc[i][j]);
 }
 
 // Cholesky of the diagonal tile (in LDS) and inverse of the factor.
@@ -251,27 +253,35 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
     }
 }
 
-// out[row] = base[row] + sign * sum_p M_p v[vcol_p]   (64-row gathers)
-__global__ __launch_bounds__(256) void k_gtask(int task0, const int32_t *row, const int64_t *tptr,
-                                               const int32_t *mat, const int32_t *vcol,
-                                               const double *mats, const double *v, const double *base,
+// out[row] += sign * sum_p M_p v[vcol_p]   (64-row gathers; one workgroup per
+// chunk of at most GCH items of a task, partial sums added atomically)
+constexpr int GCH = 4;
+__global__ __launch_bounds__(256) void k_gtask(int chunk0, const int32_t *crow, const int64_t *cbeg,
+                                               const int64_t *cend, const int32_t *mat,
+                                               const int32_t *vcol, const double *mats, const double *v,
                                                double sign, double *out) {
-  const int task = task0 + blockIdx.x;
-  const int r = row[task];
+  const int ck = chunk0 + blockIdx.x;
+  const int r = crow[ck];
   const int i = threadIdx.x & 63, part = threadIdx.x >> 6;  // 4 waves split the columns
   __shared__ double red[4][TS];
   double acc = 0;
-  for (int64_t p = tptr[task]; p < tptr[task + 1]; ++p) {
+  for (int64_t p = cbeg[ck]; p < cend[ck]; ++p) {
     const double *M = mats + (int64_t)mat[p] * TT;
     const double *vv = v + (int64_t)vcol[p] * TS;
+#pragma unroll 4
     for (int c = part; c < TS; c += 4) acc += M[i + TS * c] * vv[c];
   }
   red[part][i] = acc;
   __syncthreads();
   if (part == 0) {
     const double s = red[0][i] + red[1][i] + red[2][i] + red[3][i];
-    out[(int64_t)r * TS + i] = (base ? base[(int64_t)r * TS + i] : 0.0) + sign * s;
+    atomicAdd(out + (int64_t)r * TS + i, sign * s);
   }
+}
+
+__global__ void k_copyv(int64_t n, const double *x, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
 }
 
 __global__ void k_to_tiles(int64_t npos, const int32_t *pos2dof, const double *x, double *w) {
@@ -301,20 +311,31 @@ void up(DBuf<T> &b, const std::vector<T> &h) {
   else b.upload(h);
 }
 
-// a gather phase built per height
+// a gather phase built per height; every task is cut into chunks of at most
+// GCH items, one workgroup each
 struct PhaseBuilder {
-  std::vector<int32_t> row, mat, vcol, hptr{0};
-  std::vector<int64_t> tptr{0};
-  void task(int r) { row.push_back(r); }
+  std::vector<int32_t> crow, mat, vcol, hptr{0};
+  std::vector<int64_t> cbeg, cend;
+  int cur = -1;
+  int64_t start = 0;
+  void task(int r) { cur = r; start = (int64_t)mat.size(); }
   void item(int m, int c) { mat.push_back(m); vcol.push_back(c); }
-  void end_task() { tptr.push_back((int64_t)mat.size()); }
-  void end_height() { hptr.push_back((int32_t)row.size()); }
+  void end_task() {
+    const int64_t n = (int64_t)mat.size();
+    for (int64_t b = start; b < n; b += GCH) {
+      crow.push_back(cur);
+      cbeg.push_back(b);
+      cend.push_back(std::min<int64_t>(b + GCH, n));
+    }
+  }
+  void end_height() { hptr.push_back((int32_t)crow.size()); }
   void upload(CholTiles::Phase &p) {
     p.h_ptr = hptr;
-    up(p.row, row);
+    up(p.row, crow);
     up(p.mat, mat);
     up(p.vcol, vcol);
-    up(p.tptr, tptr);
+    up(p.tptr, cbeg);
+    up(p.tend, cend);
   }
 };
 
@@ -427,7 +448,7 @@ void chol_analyse(m3_ctx *c) {
   }
   ch.nlev = nlev;
   std::vector<int32_t> lev_cols, upd_tile, trsm_tile, trsm_col, pairs;
-  std::vector<int64_t> pair_ptr{0};
+  std::vector<int64_t> pair_beg, pair_end;
   ch.h_lev_ptr.assign(nlev + 1, 0);
   ch.h_upd_ptr.assign(nlev + 1, 0);
   ch.h_trsm_ptr.assign(nlev + 1, 0);
@@ -438,7 +459,7 @@ void chol_analyse(m3_ctx *c) {
       lev_cols.push_back(J);
       for (int32_t t = ch.h_colptr[J]; t < ch.h_colptr[J + 1]; ++t) {
         const int I = ch.h_rowtile[t];
-        upd_tile.push_back(t);
+        const int64_t p0 = (int64_t)pairs.size() / 2;
         for (auto &kt : rows[J]) {
           const int32_t tIK = tile_of(I, kt.first);
           if (tIK >= 0) {
@@ -446,7 +467,12 @@ void chol_analyse(m3_ctx *c) {
             pairs.push_back(kt.second);
           }
         }
-        pair_ptr.push_back((int64_t)pairs.size() / 2);
+        const int64_t p1 = (int64_t)pairs.size() / 2;
+        for (int64_t b = p0; b < p1; b += UCH) {
+          upd_tile.push_back(t);
+          pair_beg.push_back(b);
+          pair_end.push_back(std::min<int64_t>(b + UCH, p1));
+        }
         if (I != J) {
           trsm_tile.push_back(t);
           trsm_col.push_back(J);
@@ -582,7 +608,8 @@ void chol_analyse(m3_ctx *c) {
   up(ch.pos2dof, pos2dof);
   up(ch.lev_cols, lev_cols);
   up(ch.upd_tile, upd_tile);
-  up(ch.upd_pair_ptr, pair_ptr);
+  up(ch.upd_pair_ptr, pair_beg);
+  up(ch.upd_pair_end, pair_end);
   up(ch.upd_pairs, pairs);
   up(ch.trsm_tile, trsm_tile);
   up(ch.trsm_col, trsm_col);
@@ -619,7 +646,7 @@ void chol_factor(m3_ctx *c) {
     const int u0 = ch.h_upd_ptr[l], u1 = ch.h_upd_ptr[l + 1];
     if (u1 > u0)
       hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(256), 0, s, ch.upd_tile.p, ch.upd_pair_ptr.p,
-                         ch.upd_pairs.p, u0, ch.tiles.p);
+                         ch.upd_pair_end.p, ch.upd_pairs.p, u0, ch.tiles.p);
     const int c0 = ch.h_lev_ptr[l], c1 = ch.h_lev_ptr[l + 1];
     hipLaunchKernelGGL(k_potrf, dim3(c1 - c0), dim3(256), 0, s, ch.lev_cols.p, c0, ch.diagtile.p,
                        ch.pos2dof.p, ch.tiles.p, ch.dinv.p, ch.dinvT.p, c->errflag.p);
@@ -639,11 +666,11 @@ void chol_factor(m3_ctx *c) {
 }
 
 static void run_phase(hipStream_t s, CholTiles::Phase &p, int h, const double *mats, const double *v,
-                      const double *base, double sign, double *out) {
+                      double sign, double *out) {
   const int a = p.h_ptr[h], b = p.h_ptr[h + 1];
   if (b > a)
-    hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, a, p.row.p, p.tptr.p, p.mat.p, p.vcol.p, mats,
-                       v, base, sign, out);
+    hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, a, p.row.p, p.tptr.p, p.tend.p, p.mat.p,
+                       p.vcol.p, mats, v, sign, out);
 }
 
 void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
@@ -652,13 +679,18 @@ void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
   hipStream_t s = c->stream;
   const int64_t npos = (int64_t)ch.T * TS;
   hipLaunchKernelGGL(k_to_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.w.p);
+  hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.y.p);
+  hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.x.p);
+  // forward: y_S = X_S w_S ; w_A -= L_AS y_S
   for (int h = 0; h < ch.H; ++h) {
-    run_phase(s, ch.fdiag, h, ch.X.p, ch.w.p, nullptr, 1.0, ch.y.p);
-    run_phase(s, ch.fupd, h, ch.tiles.p, ch.y.p, ch.w.p, -1.0, ch.w.p);
+    run_phase(s, ch.fdiag, h, ch.X.p, ch.w.p, 1.0, ch.y.p);
+    run_phase(s, ch.fupd, h, ch.tiles.p, ch.y.p, -1.0, ch.w.p);
   }
+  // backward: t_S = y_S - L_AS' x_A ; x_S = X_S' t_S
+  hipLaunchKernelGGL(k_copyv, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.y.p, ch.t.p);
   for (int k = 0; k < ch.H; ++k) {
-    run_phase(s, ch.bupd, k, ch.tilesT.p, ch.x.p, ch.y.p, -1.0, ch.t.p);
-    run_phase(s, ch.bdiag, k, ch.XT.p, ch.t.p, nullptr, 1.0, ch.x.p);
+    run_phase(s, ch.bupd, k, ch.tilesT.p, ch.x.p, -1.0, ch.t.p);
+    run_phase(s, ch.bdiag, k, ch.XT.p, ch.t.p, 1.0, ch.x.p);
   }
   const int64_t n3 = 3 * cl.nnod;
   hipLaunchKernelGGL(k_zero, dim3(nblk(n3)), dim3(256), 0, s, n3, x);

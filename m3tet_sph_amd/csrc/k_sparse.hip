@@ -95,14 +95,11 @@ __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const
       const int64_t o = 3 * row;
       s = (double)x[o] * (double)y0 + (double)x[o + 1] * (double)y1 + (double)x[o + 2] * (double)y2;
     }
-    __shared__ double sh[256];
-    sh[threadIdx.x] = s;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double sh[4];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
   }
 }
 
@@ -180,19 +177,30 @@ __global__ void k_prolong(int64_t nf, const int64_t *ptr, const int32_t *col, co
 
 // restriction rc = mask_c * (P' rf); also writes the first smoother iterate
 // d1 = w1 Dinv rc of the coarse level when d1 != nullptr
+// 16 lanes cooperate on one coarse row (a coarse node gathers from up to
+// ~100 fine nodes), reduced with cross-lane shuffles.
 template <class VT, class WT>
-__global__ void k_restrict(int64_t nc, const int64_t *ptr, const int32_t *col, const WT *w,
-                           const double *mask, const VT *rf, VT *rc, const VT *dinv, VT w1, VT *d1) {
-  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (j >= nc) return;
+__global__ __launch_bounds__(256) void k_restrict(int64_t nc, const int64_t *ptr, const int32_t *col,
+                                                  const WT *w, const double *mask, const VT *rf, VT *rc,
+                                                  const VT *dinv, VT w1, VT *d1) {
+  const int64_t j = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 4;
+  const int sub = threadIdx.x & 15;
   VT a0 = 0, a1 = 0, a2 = 0;
-  for (int64_t k = ptr[j]; k < ptr[j + 1]; ++k) {
-    const int64_t f3 = 3 * (int64_t)col[k];
-    const VT wk = (VT)w[k];
-    a0 += wk * rf[f3];
-    a1 += wk * rf[f3 + 1];
-    a2 += wk * rf[f3 + 2];
+  if (j < nc) {
+    for (int64_t k = ptr[j] + sub; k < ptr[j + 1]; k += 16) {
+      const int64_t f3 = 3 * (int64_t)col[k];
+      const VT wk = (VT)w[k];
+      a0 += wk * rf[f3];
+      a1 += wk * rf[f3 + 1];
+      a2 += wk * rf[f3 + 2];
+    }
   }
+  for (int off = 8; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, 16);
+    a1 += __shfl_xor(a1, off, 16);
+    a2 += __shfl_xor(a2, off, 16);
+  }
+  if (j >= nc || sub != 0) return;
   const int64_t o = 3 * j;
   a0 *= (VT)mask[o];
   a1 *= (VT)mask[o + 1];
@@ -417,7 +425,7 @@ void prolong(hipStream_t s, LevelDev &fine, const double *dc, const double *dd, 
                      fine.P_ptr.p, fine.P_col.p, fine.P_val.p, fine.mask.p, dc, dd, df);
 }
 void restrict_(hipStream_t s, LevelDev &fine, LevelDev &coarse, const double *rf, double *rc) {
-  hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(coarse.nnod)), dim3(256), 0, s,
+  hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(16 * coarse.nnod)), dim3(256), 0, s,
                      coarse.nnod, fine.R_ptr.p, fine.R_col.p, fine.R_val.p, coarse.mask.p, rf, rc,
                      (const double *)nullptr, 0.0, (double *)nullptr);
 }
@@ -472,21 +480,22 @@ static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
     if (l == 0)
       hipLaunchKernelGGL((k_scale<VT>), dim3(nblk(n3)), dim3(256), 0, s, n3, (VT)wd[0], v.dinv, v.r0, v.d);
     // d (= w1 Dinv r0) is in v.d; rr = r0 is kept (v.r0 not overwritten)
+    const int pk = std::is_same<VT, double>::value ? 0 : 1;
     pick_K(l, [&](auto K) {
-      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wd[1], (VT *)nullptr, v.dn);
-      sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wd[2], (VT *)nullptr, v.dd);
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wd[1], (VT *)nullptr, v.dn); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wd[2], (VT *)nullptr, v.dd); }
       sweep(s, K, lv.nnod, v.dd, v.r0, v.dinv, 0.0, v.r1, (VT *)nullptr);
     });
     LevelDev &cl = c->L[l + 1];
     VecSet<VT> vc = vs(cl);
     const bool coarsest = (l + 1 == nm - 1);
     if constexpr (std::is_same<VT, double>::value)
-      hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(cl.nnod)), dim3(256), 0, s, cl.nnod,
+      hipLaunchKernelGGL((k_restrict<double, double>), dim3(nblk(16 * cl.nnod)), dim3(256), 0, s, cl.nnod,
                          lv.R_ptr.p, lv.R_col.p, lv.R_val.p, cl.mask.p, v.r1, vc.r0,
                          coarsest ? (const double *)nullptr : vc.dinv, WDOWN[0],
                          coarsest ? (double *)nullptr : vc.d);
     else
-      hipLaunchKernelGGL((k_restrict<float, float>), dim3(nblk(cl.nnod)), dim3(256), 0, s, cl.nnod,
+      hipLaunchKernelGGL((k_restrict<float, float>), dim3(nblk(16 * cl.nnod)), dim3(256), 0, s, cl.nnod,
                          lv.R_ptr.p, lv.R_col.p, lv.R_val32.p, cl.mask.p, v.r1, vc.r0,
                          coarsest ? (const float *)nullptr : vc.dinv, (float)WDOWN[0],
                          coarsest ? (float *)nullptr : vc.d);
@@ -515,10 +524,11 @@ static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
     else
       hipLaunchKernelGGL((k_prolong<float, float>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.nnod,
                          lv.P_ptr.p, lv.P_col.p, lv.P_val32.p, lv.mask.p, vc.d, v.dd, v.d);
+    const int pk = std::is_same<VT, double>::value ? 0 : 1;
     pick_K(l, [&](auto K) {
-      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[0], (VT *)nullptr, v.dn);
-      sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[1], (VT *)nullptr, v.d);
-      sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[2], (VT *)nullptr, v.dn);
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[0], (VT *)nullptr, v.dn); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[1], (VT *)nullptr, v.d); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[2], (VT *)nullptr, v.dn); }
     });
     // result of the last sweep is in dn: swap so that d holds it
     if constexpr (std::is_same<VT, double>::value) std::swap(lv.d, lv.dn);

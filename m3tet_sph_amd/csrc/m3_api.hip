@@ -621,7 +621,15 @@ static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, 
   }
   if (itPat > o->itmax_Pat) itPat = o->itmax_Pat;
   rhs_of_P();
-  nitU += vel_solve(Uf, Rhs, tol_U_abs, true);
+  if (o->rtol_U_final > 0) {
+    const double normf = std::sqrt(m3::dot(c, Rhs, Rhs, n3) / (double)f.nfree);
+    const double tol = std::min(tol_U_abs, o->rtol_U_final * normf);
+    auto t0 = clk::now();
+    nitU += m3::mgpcg_run(c, Uf, Rhs, tol, true, std::max(itmax_U, o->itmax_U_final), prec, nullptr);
+    t_vel += secs(t0, clk::now());
+  } else {
+    nitU += vel_solve(Uf, Rhs, tol_U_abs, true);
+  }
   // U = Uf + U0 (rotated frame), P
   m3::copy(s, Uf, Y, n3);
   m3::axpy(s, 1.0, c->U0.p, Y, n3);
@@ -723,6 +731,94 @@ int m3_get_vector(m3_ctx *c, int32_t which, int32_t level, double *out) {
         for (int64_t i = 0; i < c->nV; ++i) out[i] = 1.0 / out[i];
         break;
       default: throw M3Error(M3_E_ARG, "bad vector id");
+    }
+  });
+}
+
+int m3_residual(m3_ctx *c, const double *U, const double *P, double *out) {
+  return guard(c, [&] {
+    M3_REQUIRE(c->assembled, M3_E_STATE, "m3_assemble must be called first");
+    hipStream_t s = c->stream;
+    LevelDev &f = c->L[0];
+    const int64_t n3 = 3 * f.nnod, nV = f.nV;
+    DBuf<double> u, uf, p, fU, KU, rP;
+    u.upload(U, n3);
+    p.upload(P, nV);
+    uf.alloc(n3);
+    fU.alloc(n3);
+    KU.alloc(n3);
+    rP.alloc(nV);
+    // f_U = mask (Fb + GG P - bc_rhs)
+    m3::spmv_G(s, c->G, p.p, fU.p, f.mask.p);
+    m3::copy(s, c->Fb.p, KU.p, n3);
+    m3::axpy(s, -1.0, c->bc_rhs.p, KU.p, n3);
+    m3::scale_mask(s, KU.p, f.mask.p, KU.p, n3);
+    m3::axpy(s, 1.0, KU.p, fU.p, n3);
+    const double nf = (double)f.nfree;
+    const double normf = std::sqrt(m3::dot(c, fU.p, fU.p, n3) / nf);
+    // r_U = f_U - K1 (mask U)
+    m3::scale_mask(s, u.p, f.mask.p, uf.p, n3);
+    m3::spmv_K(s, f, uf.p, KU.p);
+    m3::scale_mask(s, KU.p, f.mask.p, KU.p, n3);
+    m3::axpy(s, -1.0, KU.p, fU.p, n3);
+    const double normr = std::sqrt(m3::dot(c, fU.p, fU.p, n3) / nf);
+    // r_P = -GG' U, mean removed
+    m3::spmv_GT(s, c->GT, u.p, rP.p);
+    m3::add_scalar(s, rP.p, -m3::mean(c, rP.p, nV), nV);
+    out[0] = normr;
+    out[1] = normf;
+    out[2] = std::sqrt(m3::dot(c, rP.p, rP.p, nV) / (double)nV);
+  });
+}
+
+int m3_profile(m3_ctx *c, int32_t enable) {
+  return guard(c, [&] {
+    c->prof = enable != 0;
+    if (enable) {
+      for (int k = 0; k < 2; ++k) {
+        c->prof_ev[k].clear();
+        c->prof_acc[k] = 0;
+        c->prof_n[k] = 0;
+      }
+    }
+  });
+}
+
+int m3_profile_read(m3_ctx *c, double *out, int32_t nout) {
+  return guard(c, [&] {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2; ++k) {
+      for (auto &ev : c->prof_ev[k]) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev.first, ev.second));
+        c->prof_acc[k] += ms * 1e-3;
+        c->prof_n[k] += 1;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+      }
+      c->prof_ev[k].clear();
+    }
+    const double v[4] = {(double)c->prof_n[0], c->prof_acc[0], (double)c->prof_n[1], c->prof_acc[1]};
+    for (int i = 0; i < nout && i < 4; ++i) out[i] = v[i];
+  });
+}
+
+int m3_get_operator(m3_ctx *c, int32_t level, int64_t *rowptr, int32_t *col, double *val) {
+  return guard(c, [&] {
+    M3_REQUIRE(level >= 0 && level < c->nmg, M3_E_ARG, "bad level");
+    M3_REQUIRE(level == 0 ? c->assembled : c->restricted, M3_E_STATE, "operator not built");
+    Sell &K = c->L[level].K;
+    const std::vector<double> v = K.val.to_host();
+    int64_t q = 0;
+    rowptr[0] = 0;
+    for (int64_t r = 0; r < K.nrow; ++r) {
+      const int64_t base = K.h_slice_ptr[r >> 6] + (r & 63);
+      for (int k = 0; k < K.h_rowlen[r]; ++k, ++q) {
+        const int64_t slot = base + (int64_t)k * 64;
+        col[q] = K.h_col[slot];
+        for (int e = 0; e < 9; ++e) val[9 * q + e] = v[(size_t)e * K.nslot + slot];
+      }
+      rowptr[r + 1] = q;
     }
   });
 }

@@ -120,7 +120,7 @@ struct CholTiles {
   int nlev = 0;
   std::vector<int32_t> h_lev_ptr, h_upd_ptr, h_trsm_ptr;
   DBuf<int32_t> lev_cols, upd_tile, upd_pairs, trsm_tile, trsm_col;
-  DBuf<int64_t> upd_pair_ptr;
+  DBuf<int64_t> upd_pair_ptr, upd_pair_end;  // per update chunk: pair range
   // supernodes: explicit inverse of each block's diagonal part
   int nblock = 0, H = 0, maxm = 0;
   int64_t nX = 0;
@@ -131,9 +131,9 @@ struct CholTiles {
   DBuf<int32_t> inv_pairs;                   // (L tile, X tile) pairs
   // solves: four gather-task phases per height
   struct Phase {
-    std::vector<int32_t> h_ptr;  // per height: task range
+    std::vector<int32_t> h_ptr;  // per height: chunk range
     DBuf<int32_t> row, mat, vcol;
-    DBuf<int64_t> tptr;
+    DBuf<int64_t> tptr, tend;    // chunk item ranges
   } fdiag, fupd, bupd, bdiag;
   DBuf<double> w, y, t, x;       // T*64 solve vectors
 };
@@ -191,7 +191,32 @@ struct m3_ctx {
   DBuf<double> Uv, Zv, Yv, Rhs;
   DBuf<double> pr, prlast, pd, pq, pSq, pP, pMd;
   m3_phys phys{};
+  // kernel timing (fine-level sweeps), events recorded around launches
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[2];
+  double prof_acc[2] = {0, 0};
+  int64_t prof_n[2] = {0, 0};
 };
+
+namespace m3 {
+// bracket a fine-level sweep launch with events when profiling is on
+struct ProfScope {
+  m3_ctx *c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(m3_ctx *c_, int kind_, bool active) : c(c_), kind(kind_) {
+    if (!active || !c->prof) return;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    (void)hipEventRecord(b, c->stream);
+    c->prof_ev[kind].push_back({a, b});
+  }
+};
+}  // namespace m3
 
 // ---------------------------------------------------------------- launchers
 namespace m3 {

@@ -1,0 +1,259 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+c0 (17,334 velocity dofs, 2-level MG): every operator and solver stage is
+compared with the oracle on the same seeded inputs.  c1 (1,013,958 dofs,
+3-level MG): size-independent properties (Galerkin identity, symmetry,
+residual reduction) and the complete solve to relative residual 1e-6.
+Floating-point tolerances: 1e-12 relative (Frobenius / L2) for assembled
+operators, 1e-10 for solver outputs compared with the oracle iterating the
+same algorithm, 1e-8 against the exact discrete solution.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from tests.conftest import relerr
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ora(c0, c0_inputs):
+    from oracle import assembly as A, stokes as S
+    i = c0_inputs
+    KK, Fb, GG, MM, _ = A.assemble_opt(c0, i["Visc"], i["Dens"], i["phys"])
+    RR = S.make_rotation_matrix(c0.GCOORD)
+    St = S.rotate_and_lift(KK, GG, Fb, RR, i["ifix"], i["vfix"], np.zeros(c0.nVnod))
+    ip, ci, w = c0.transfer(0)
+    Ic = S.ic2f_dof(ip, ci, w, c0.levels[1].nnod)
+    KKl, IcU = S.restrict_stiffmat(St["K1"], St["ifree"], [Ic], [3 * L.nnod for L in c0.levels])
+    return dict(KK=KK, Fb=Fb, GG=GG, MM=MM, St=St, KKl=KKl, IcU=IcU, chol=S.Chol(KKl[-1]))
+
+
+@pytest.fixture(scope="module")
+def gpu(c0, c0_inputs):
+    from m3tet_sph_amd import api
+    log("GPU context c0")
+    ctx = api.StokesContext(c0)
+    i = c0_inputs
+    ctx.set_bc(i["ifix"], i["vfix"])
+    ctx.assemble(i["Visc"], i["Dens"], i["phys"])
+    ctx.restrict_stiffmat()
+    ctx.cholesky_factorization()
+    yield ctx
+    ctx.close()
+
+
+def test_element_matrices(c0, c0_inputs, gpu):
+    from oracle import assembly as A
+    i = c0_inputs
+    got = gpu.element_matrices(i["Visc"], i["Dens"], i["phys"])
+    ref = A.element_matrices(c0, i["Visc"], i["Dens"], i["phys"])
+    for g, r, n in zip(got, ref, "KGMF"):
+        log("element", n, relerr(g, r))
+        assert relerr(g, r) < 1e-12
+        # per class as well
+        for kind, els in c0.classes.items():
+            assert relerr(g[els], r[els]) < 1e-12, (n, kind)
+
+
+def test_element_matrices_221(c0, c0_inputs, gpu):
+    """'221' D-matrix variant (C2 = 0) of the same kernels."""
+    from oracle import elements as E, assembly as A
+    i = c0_inputs
+    K = gpu.element_matrices(i["Visc"], i["Dens"], i["phys"], type_Dmat="221")[0]
+    # closed form: '221' - '23rd' = (2/3) (div N)(div N) terms -> compare via std B'DB
+    D221 = np.diag([2.0, 2, 2, 1, 1, 1])
+    saved = A.D23.copy()
+    try:
+        A.D23[:] = D221
+        Kr = A.element_matrices(c0, i["Visc"], i["Dens"], i["phys"], kernel=A.class_kernel_std)[0]
+    finally:
+        A.D23[:] = saved
+    assert relerr(K, Kr) < 1e-12
+
+
+def test_reference_named_assembly(c0, c0_inputs, gpu, ora):
+    from m3tet_sph_amd import api
+    i = c0_inputs
+    c0._m3ctx = gpu
+    KK, Fb, GG, MM, Fpdil, _, _ = api.assembly_stokes_sph_3d_double_jacobian(
+        dict(Visc=i["Visc"], Dens=i["Dens"]), c0, dict(g=i["phys"]["g"]),
+        dict(Bscale=i["phys"]["Bscale"], Dens0=i["phys"]["Dens0"]), dict(nip=14, type_Dmat="23rd"))
+    assert abs(KK - ora["KK"]).max() < 1e-12 * abs(ora["KK"]).max()
+    assert abs(GG - ora["GG"]).max() < 1e-12 * abs(ora["GG"]).max()
+    assert abs(MM - ora["MM"]).max() < 1e-12 * abs(ora["MM"]).max()
+    assert relerr(Fb, ora["Fb"]) < 1e-12
+    assert Fpdil == 0.0
+
+
+def test_global_operators(c0, gpu, ora):
+    St = ora["St"]
+    rng = np.random.default_rng(0)
+    ifree = St["ifree"]
+    mask = gpu.get_vector(3, 0)
+    assert np.array_equal(np.nonzero(mask)[0], ifree)
+    x = rng.standard_normal(3 * c0.nnod) * mask
+    y = gpu.apply(0, 0, x)
+    assert relerr(y[ifree], St["K1"] @ x[ifree]) < 1e-12
+    assert relerr(gpu.get_vector(0), St["Fb"]) < 1e-12
+    assert relerr(gpu.get_vector(1), St["bc_rhs"]) < 1e-12
+    q = rng.standard_normal(c0.nVnod)
+    assert relerr(gpu.apply(1, 0, q), St["GG"] @ q) < 1e-12
+    z = rng.standard_normal(3 * c0.nnod)
+    assert relerr(gpu.apply(2, 0, z), St["GG"].T @ z) < 1e-12
+    MMf = ora["MM"] + ora["MM"].T - __import__("scipy.sparse", fromlist=["diags"]).diags(ora["MM"].diagonal())
+    assert relerr(gpu.apply(3, 0, q), MMf @ q) < 1e-12
+    assert relerr(gpu.get_vector(4), np.asarray(MMf.sum(axis=1)).ravel()) < 1e-12
+    assert relerr(gpu.get_vector(2, 0)[ifree], ora["KKl"][0]) < 1e-12
+
+
+def test_transfers_and_galerkin_levels(c0, gpu, ora):
+    St, KKl, IcU = ora["St"], ora["KKl"], ora["IcU"]
+    ifr = St["ifree"]
+    n3c = 3 * c0.levels[1].nnod
+    fr = ifr[ifr < n3c]
+    rng = np.random.default_rng(1)
+    xc = np.zeros(n3c)
+    xc[fr] = rng.standard_normal(len(fr))
+    yf = gpu.apply(4, 0, xc)
+    assert relerr(yf[ifr], IcU[0] @ xc[fr]) < 1e-13
+    xf = np.zeros(3 * c0.nnod)
+    xf[ifr] = rng.standard_normal(len(ifr))
+    yc = gpu.apply(5, 0, xf)
+    assert relerr(yc[fr], IcU[0].T @ xf[ifr]) < 1e-13
+    yk = gpu.apply(0, 1, xc)
+    assert relerr(yk[fr], KKl[1] @ xc[fr]) < 1e-12
+
+
+def test_cholesky_solve(c0, gpu, ora):
+    ifr = ora["St"]["ifree"]
+    n3c = 3 * c0.levels[1].nnod
+    fr = ifr[ifr < n3c]
+    b = np.zeros(n3c)
+    b[fr] = np.random.default_rng(5).standard_normal(len(fr))
+    x = gpu.cholesky_solve(b)
+    assert relerr(x[fr], ora["chol"].solve(b[fr])) < 1e-11
+    assert np.all(x[np.setdiff1d(np.arange(n3c), fr)] == 0)
+
+
+def test_mgpcg_iterates_like_the_oracle(c0, gpu, ora):
+    from oracle import stokes as S
+    St = ora["St"]
+    ifr = St["ifree"]
+    b = np.zeros(3 * c0.nnod)
+    b[ifr] = St["Rhs"]
+    x, it, hist = gpu.mgpcg(b, tol=1e-10, is_abs_tol=False, itmax=50)
+    xo, ito, histo = S.mgpcg_solver_p(St["K1"], ora["KKl"], ora["chol"], St["Rhs"], np.zeros(len(ifr)),
+                                      ora["IcU"], 1e-10, False, 50)
+    log("mgpcg", it, ito, hist[-1], histo[-1])
+    assert it == ito
+    assert relerr(hist, histo) < 1e-8
+    assert relerr(x[ifr], xo) < 1e-10
+
+
+def test_patera_iterates_like_the_oracle(c0, c0_inputs, gpu, ora):
+    from oracle import stokes as S
+    i = c0_inputs
+    U, P, info = gpu.patera(rtol_Pat=1e-8)
+    Uo, Po, infoo, _, _, _ = S.solve_reference(c0, ora["KK"], ora["Fb"], ora["GG"], ora["MM"],
+                                               i["ifix"], i["vfix"], rtol_Pat=1e-8)
+    log("patera", info, infoo["nitP"], infoo["nitU"], infoo["nitZ"])
+    assert (info["nitP"], info["nitU"], info["nitZ"]) == (infoo["nitP"], infoo["nitU"], infoo["nitZ"])
+    assert relerr(U, Uo) < 1e-10
+    assert relerr(P, Po) < 1e-10
+
+
+def test_patera_matches_exact_discrete_solution(c0, gpu, ora):
+    """Plain FP64 GPU solve vs the exact discrete Stokes solution: 1e-8."""
+    from oracle import stokes as S
+    St = ora["St"]
+    U, P, info = gpu.patera(rtol_Pat=1e-11, rtol_U=1e-3, itmax_U=60, rtol_U_final=1e-12, itmax_U_final=80)
+    Ud, Pd = S.direct_saddle_solve(St["K1"], St["GG"], St["Fb"], St["bc_rhs"], St["ifree"], St["U0"])
+    log("exact", relerr(U, Ud), relerr(P - P.mean(), Pd - Pd.mean()))
+    assert relerr(U, Ud) < 1e-8
+    assert relerr(P - P.mean(), Pd - Pd.mean()) < 1e-8
+
+
+def test_flow3d_mirror_returns_cartesian_velocity(c0, c0_inputs, gpu):
+    from m3tet_sph_amd import api
+    from m3tet_sph_amd.fields import rotation_frames
+    i = c0_inputs
+    c0._m3ctx = gpu
+    gpu._bc_key = None
+    VAR, info = api.flow3d_patera_sph(dict(Visc=i["Visc"], Dens=i["Dens"]), c0, None,
+                                      dict(ifix=i["ifix"], vfix=i["vfix"]), None, dict(g=i["phys"]["g"]),
+                                      dict(Bscale=i["phys"]["Bscale"], Dens0=i["phys"]["Dens0"]))
+    R = rotation_frames(c0.GCOORD)
+    Us = np.stack([VAR["Uth"], VAR["Uph"], VAR["Ur"]], 1)
+    Ux = np.einsum("nij,nj->ni", R, Us)
+    assert np.allclose(Ux[:, 0], VAR["Ux"]) and np.allclose(Ux[:, 2], VAR["Uz"])
+    # prescribed dofs hold their values
+    U = Us.ravel()
+    assert np.allclose(U[i["ifix"]], i["vfix"], atol=1e-14)
+    assert info["nitP"] >= 3
+
+
+def test_error_paths(c0, c0_inputs, gpu):
+    from m3tet_sph_amd import api
+    i = c0_inputs
+    bad = i["Visc"].copy()
+    bad[:] = -1.0
+    with pytest.raises(api.M3Error, match="negative viscosity"):
+        gpu.assemble(bad, i["Dens"], i["phys"])
+    gpu.assemble(i["Visc"], i["Dens"], i["phys"])
+    with pytest.raises(api.M3Error, match="operators are not ready"):
+        gpu.mgpcg(np.zeros(3 * c0.nnod))
+    gpu.restrict_stiffmat()
+    gpu.cholesky_factorization()
+    b = np.zeros(3 * c0.nnod)
+    b[5] = np.nan
+    with pytest.raises(api.M3Error, match="NaN"):
+        gpu.mgpcg(b, itmax=3)
+
+
+# ----------------------------------------------------------------- c1 (1M dofs)
+
+@pytest.fixture(scope="module")
+def c1ctx():
+    from m3tet_sph_amd import mesh as M, fields as F, api
+    log("GPU context c1")
+    m = M.build_config("c1")
+    inp = F.make_inputs(m, visc_contrast=1e4)
+    ctx = api.StokesContext(m)
+    ctx.set_bc(inp["ifix"], inp["vfix"])
+    yield m, inp, ctx
+    ctx.close()
+
+
+def test_c1_galerkin_identity_and_symmetry(c1ctx):
+    m, inp, ctx = c1ctx
+    ctx.assemble(inp["Visc"], inp["Dens"], inp["phys"])
+    ctx.restrict_stiffmat()
+    rng = np.random.default_rng(2)
+    for l in range(m.nmg - 1):
+        mc = ctx.get_vector(3, l + 1)
+        x, y = rng.standard_normal(len(mc)) * mc, rng.standard_normal(len(mc)) * mc
+        Px, Py = ctx.apply(4, l, x), ctx.apply(4, l, y)
+        lhs = Px @ ctx.apply(0, l, Py)
+        rhs = x @ ctx.apply(0, l + 1, y)
+        log("galerkin", l, lhs, rhs)
+        assert abs(lhs - rhs) < 1e-11 * abs(rhs)
+        mf = ctx.get_vector(3, l)
+        u, v = rng.standard_normal(len(mf)) * mf, rng.standard_normal(len(mf)) * mf
+        a, b = u @ ctx.apply(0, l, v), v @ ctx.apply(0, l, u)
+        assert abs(a - b) < 1e-12 * abs(a)
+
+
+def test_c1_full_solve_reaches_tolerance(c1ctx):
+    from m3tet_sph_amd import api
+    m, inp, ctx = c1ctx
+    U, P, info = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], rtol_Pat=1e-6, rtol_U_final=5e-7)
+    rU, fU, rP = api.residual(ctx, U, P)
+    log("c1", info, rU / fU, rP / info["norm_rP0"])
+    assert rU / fU < 1e-6
+    assert rP / info["norm_rP0"] < 1e-6
+    assert np.allclose(U[inp["ifix"]], inp["vfix"], atol=1e-14)

@@ -1,0 +1,78 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point declared in include/m3stokes.h; the mesh set-up layer
+produces the reference's counts and invariants."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.conftest import ROOT
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "m3stokes.h")).read()
+    return sorted(set(re.findall(r"\b(m3_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_header_symbols():
+    from m3tet_sph_amd import api
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(api.LIB, s), s
+        assert isinstance(getattr(api.LIB, s), ctypes._CFuncPtr)
+
+
+def test_create_reports_error_without_device():
+    from m3tet_sph_amd import api
+    h = ctypes.c_void_p()
+    rc = api.LIB.m3_create(ctypes.byref(h), 0)
+    if rc == 0:
+        api.LIB.m3_destroy(h)
+        pytest.skip("a GPU is visible")
+    assert rc in (2, 1)
+    assert api.LIB.m3_last_error(h)
+    api.LIB.m3_destroy(h)
+
+
+def test_null_context_is_rejected():
+    from m3tet_sph_amd import api
+    assert api.LIB.m3_assemble(None, None, None, None) == 1
+    assert api.LIB.m3_last_error(None) == b"null context"
+
+
+@pytest.mark.parametrize("name,nel,nnod,nV", [
+    ("c0", 3840, 5778, 810),
+    ("c1", 245760, 337986, 43554),
+])
+def test_config_counts(name, nel, nnod, nV):
+    from m3tet_sph_amd import mesh as M
+    m = M.build_config(name)
+    assert (m.nel, m.nnod, m.nVnod) == (nel, nnod, nV)
+    assert M.tet_volume6(m.GCOORD, m.levels[0].EL2NOD[:, :4]).min() > 0
+    cls = np.concatenate([v for v in m.classes.values()])
+    assert np.array_equal(np.sort(cls), np.arange(m.nel))
+    for l in range(m.nmg - 1):   # coarse vertices are the fine P2 nodes
+        assert m.levels[l].nVnod == m.levels[l + 1].nnod
+        assert len(m.levels[l].EL2NOD) == 8 * len(m.levels[l + 1].EL2NOD)
+
+
+def test_cone_classes_follow_the_rule(c0):
+    th = c0.GCOORD_SPH[:, 0]
+    tc = np.pi / 4
+    inc = (th <= tc) | (th >= np.pi - tc)
+    cnt = inc[c0.levels[0].EL2NOD[:, :4]].sum(1)
+    assert np.all(cnt[c0.classes["in_cone_iso"]] > 0) and np.all(cnt[c0.classes["in_cone_iso"]] < 3)
+    assert np.all(cnt[c0.classes["in_cone_no_iso"]] >= 3)
+    assert np.all(cnt[c0.classes["out_cone_no_cross_2pi"]] == 0)
+
+
+def test_bcs(c0, c0_inputs):
+    ifix, vfix = c0_inputs["ifix"], c0_inputs["vfix"]
+    pid = c0.PointID
+    assert len(ifix) == (pid == 301).sum() + 3 * (pid == 306).sum()
+    surf = np.nonzero(pid == 306)[0]
+    assert np.all(np.isin(3 * surf + 2, ifix))
+    assert np.abs(vfix).max() > 0.1            # plate velocities of unit rate poles

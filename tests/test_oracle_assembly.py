@@ -1,0 +1,161 @@
+"""Pins of the oracle assembly, transfer and Galerkin restatements (CPU)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import assembly as A, elements as E, stokes as S, transfer as TR
+from tests.conftest import relerr
+
+
+@pytest.fixture(scope="module")
+def elmats(c0, c0_inputs):
+    i = c0_inputs
+    return A.element_matrices(c0, i["Visc"], i["Dens"], i["phys"])
+
+
+def test_opt_equals_std(c0, c0_inputs, elmats):
+    """Two independent restatements agree (the reference's own std-vs-opt
+    cross-check, assembly_stokes_sph_3d_double_jacobian.m:66-72)."""
+    i = c0_inputs
+    std = A.element_matrices(c0, i["Visc"], i["Dens"], i["phys"], kernel=A.class_kernel_std)
+    for a, b in zip(elmats, std):
+        assert relerr(a, b) < 1e-13
+
+
+def test_planted_sign_error_is_caught(c0, c0_inputs, monkeypatch):
+    """A wrong un-rotation sign table in the 'opt' restatement must show up
+    against the 'std' restatement (which rotates with matrix products)."""
+    i = c0_inputs
+    good = A._sign_rows_z()
+    monkeypatch.setattr(A, "_sign_rows_z", lambda: good[1:])
+    els = c0.classes["out_cone_cross_2pi"][:16]
+    T = E.sf_tables()
+    K1 = A.class_kernel_opt("out_cone_cross_2pi", els, c0, i["Visc"], i["Dens"], i["phys"], T)[0]
+    K2 = A.class_kernel_std("out_cone_cross_2pi", els, c0, i["Visc"], i["Dens"], i["phys"], T)[0]
+    assert relerr(K1, K2) > 1e-3
+
+
+def _full(Kp):
+    n = Kp.shape[0]
+    K = np.zeros((n, 30, 30))
+    r, c = np.tril_indices(30)
+    o = np.lexsort((r, c))
+    K[:, r[o], c[o]] = Kp
+    K[:, c[o], r[o]] = Kp
+    return K
+
+
+def test_element_invariants(elmats):
+    K, G, M, F = elmats
+    Kf = _full(K[:400])
+    # rigid translations are in the kernel of K and of G' (sum_a grad N_a = 0)
+    for c in range(3):
+        t = np.zeros(30)
+        t[c::3] = 1.0
+        assert np.abs(Kf @ t).max() < 1e-10 * np.abs(Kf).max()
+        Gm = G[:400].reshape(-1, 4, 30)
+        assert np.abs(Gm @ t).max() < 1e-12 * np.abs(Gm).max()
+    ev = np.linalg.eigvalsh(Kf)
+    assert ev.min() > -1e-12 * ev.max()
+    assert np.all(M[:, [0, 4, 7, 9]] > 0)
+
+
+def test_shell_volume(c0):
+    """Sum of the integration weights equals the shell volume (to the
+    geometric error of the straight-in-spherical elements)."""
+    one = np.ones(c0.nnod)
+    M = A.element_matrices(c0, one, one, dict(g=-1.0, Bscale=1.0, Dens0=0.0))[2]
+    vol = (M[:, [0, 4, 7, 9]].sum() + 2 * M[:, [1, 2, 3, 5, 6, 8]].sum())
+    exact = 4 * np.pi / 3 * (1 - 0.546 ** 3)
+    assert abs(vol / exact - 1) < 5e-5
+
+
+def test_net_buoyancy_small(c0):
+    """A uniform density anomaly on a closed shell exerts no net force."""
+    one = np.ones(c0.nnod)
+    F = A.element_matrices(c0, one, 2 * one, dict(g=-1.0, Bscale=1.0, Dens0=1.0))[3]
+    net = F.reshape(-1, 10, 3).sum(axis=(0, 1))
+    assert np.linalg.norm(net) < 1e-3 * np.abs(F).sum() / 30
+
+
+def test_transfer_matches_literal_restatement(c0):
+    """Product transfer (from refinement tables) == literal
+    intermesh_transfer_matrix on the straight geometry."""
+    for l in range(c0.nmg - 1):
+        N, cols = TR.intermesh_transfer_matrix(c0, l)
+        ip, ci, w = c0.transfer(l)
+        nf = len(ip) - 1
+        dense = sp.csr_matrix((w, ci, ip), shape=(nf, c0.levels[l + 1].nnod)).toarray()
+        lit = np.zeros_like(dense)
+        for k in range(10):
+            np.add.at(lit, (np.arange(nf), cols[:, k]), N[:, k])
+        assert np.array_equal(dense, lit)
+
+
+def test_transfer_interpolates_p2_fields_exactly(c0):
+    """The reference's own x*y check idea (intermesh_transfer_matrix.m:64-77)
+    on the straight geometry, where P2 fields are reproduced exactly."""
+    Xc, Xf = TR.straight_coords(c0, 1)
+    ip, ci, w = c0.transfer(0)
+    P = sp.csr_matrix((w, ci, ip), shape=(len(ip) - 1, len(Xc)))
+    for f in (lambda X: X[:, 0] * X[:, 1], lambda X: X[:, 2] ** 2 - X[:, 0], lambda X: 1 + 0 * X[:, 0]):
+        assert np.abs(P @ f(Xc) - f(Xf[:len(ip) - 1])).max() < 1e-13
+
+
+def test_nesting_rows_lie_on_parent(c0):
+    ip, ci, w = c0.transfer(0)
+    EL, ELc = c0.levels[0].EL2NOD, c0.levels[1].EL2NOD
+    for e in range(0, len(EL), 7):
+        parent = set(ELc[e // 8])
+        for n in EL[e]:
+            assert set(ci[ip[n]:ip[n + 1]]) <= parent
+
+
+def element_local_rap(mesh, Kel, mask_f, mask_c):
+    """K_c = sum over children (Q (x) I3)' K_e (Q (x) I3), coarse dofs masked."""
+    from m3tet_sph_amd.mesh import child_interp_tables
+    Q = child_interp_tables()
+    ELc = mesh.levels[1].EL2NOD
+    ctype = mesh.child_types(0)
+    nc = 3 * mesh.levels[1].nnod
+    rows, cols, vals = [], [], []
+    dof_c = (3 * ELc[:, :, None] + np.arange(3)).reshape(len(ELc), 30)
+    for e in range(len(Kel)):
+        q = np.kron(Q[ctype[e] // 8, ctype[e] % 8], np.eye(3))
+        Ke = q.T @ Kel[e] @ q
+        d = dof_c[e // 8]
+        m = mask_c[d]
+        Ke = Ke * np.outer(m, m)
+        rows.append(np.repeat(d, 30))
+        cols.append(np.tile(d, 30))
+        vals.append(Ke.ravel())
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(nc, nc))
+
+
+def test_element_local_rap_equals_global_galerkin(c0, c0_inputs):
+    i = c0_inputs
+    KK, Fb, GG, MM, _ = A.assemble_opt(c0, i["Visc"], i["Dens"], i["phys"])
+    RR = S.make_rotation_matrix(c0.GCOORD)
+    St = S.rotate_and_lift(KK, GG, Fb, RR, i["ifix"], i["vfix"], np.zeros(c0.nVnod))
+    ip, ci, w = c0.transfer(0)
+    Ic = S.ic2f_dof(ip, ci, w, c0.levels[1].nnod)
+    KKl, IcU = S.restrict_stiffmat(St["K1"], St["ifree"], [Ic], [3 * L.nnod for L in c0.levels])
+    # element matrices rotated to nodal frames and masked
+    Kp = A.element_matrices(c0, i["Visc"], i["Dens"], i["phys"])[0]
+    from m3tet_sph_amd.fields import rotation_frames
+    R = rotation_frames(c0.GCOORD)
+    EL = c0.levels[0].EL2NOD
+    mask = np.ones(3 * c0.nnod)
+    mask[i["ifix"]] = 0
+    Kf = _full(Kp)
+    Rb = np.zeros((len(EL), 30, 30))
+    for a in range(10):
+        Rb[:, 3 * a:3 * a + 3, 3 * a:3 * a + 3] = R[EL[:, a]]
+    Kr = np.transpose(Rb, (0, 2, 1)) @ Kf @ Rb
+    dof = (3 * EL[:, :, None] + np.arange(3)).reshape(len(EL), 30)
+    mm = mask[dof]
+    Kr = Kr * mm[:, :, None] * mm[:, None, :]
+    Kc = element_local_rap(c0, Kr, mask, mask[:3 * c0.levels[1].nnod])
+    fr = St["ifree"][St["ifree"] < 3 * c0.levels[1].nnod]
+    assert relerr(Kc[fr][:, fr].toarray(), KKl[1].toarray()) < 1e-13
