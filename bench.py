@@ -128,16 +128,30 @@ def main():
         rU, fU, rP = api.residual(ctx, U, P)
         return max(rU / fU, rP / rP_cold), rU / fU, rP / rP_cold
 
-    for _ in range(args.warmup):
-        solve(args.form)
-    t0 = time.perf_counter()
-    infos = []
-    for _ in range(args.steps):
-        U, P, info = solve(args.form)
-        infos.append(info)
-    dt = time.perf_counter() - t0
+    def timed(form):
+        for _ in range(args.warmup):
+            solve(form)
+        t0 = time.perf_counter()
+        infos = []
+        for _ in range(args.steps):
+            U, P, info = solve(form)
+            infos.append(info)
+        return (time.perf_counter() - t0), U, P, infos
+
+    plain = None
+    if args.form != "fp64":
+        dtp, Up, Pp, infp = timed("fp64")
+        plain = dict(solves_per_s=round(args.steps / dtp, 4), ms_per_solve=round(1e3 * dtp / args.steps, 2),
+                     rel_res=float(f"{rel_res(Up, Pp)[0]:.3e}"),
+                     nitZ=infp[-1]["nitZ"], nitU=infp[-1]["nitU"], nitP=infp[-1]["nitP"])
+        log(f"plain fp64: {plain}")
+    dt, U, P, infos = timed(args.form)
     rr, rru, rrp = rel_res(U, P)
     ms = 1e3 * dt / args.steps
+    err_vs_fp64 = None
+    if plain is not None:
+        err_vs_fp64 = dict(U=float(f"{np.linalg.norm(U - Up) / np.linalg.norm(Up):.3e}"),
+                           P=float(f"{np.linalg.norm(P - Pp) / np.linalg.norm(Pp):.3e}"))
     n_cg = int(np.mean([i["nitU"] + i["nitZ"] for i in infos]))
     log(f"{args.steps} solves in {dt:.3f}s -> {ms:.1f} ms/solve; rel_res={rr:.2e} "
         f"(U {rru:.2e}, P {rrp:.2e}); last info {infos[-1]}")
@@ -179,7 +193,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f64+f32+bf16",
+        "dtype": "f64" if args.form == "fp64" else "f64 outer, f32/bf16 V-cycle",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: spherical-shell Stokes, {mesh.nel} P2 tets, "
                                f"{3 * mesh.nnod} velocity dofs, {mesh.nVnod} pressure dofs, "
@@ -192,6 +206,9 @@ def main():
                       ("t_assemble", "t_restrict", "t_factor", "t_pressure", "t_velocity")},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "plain_fp64": plain,
+        "speedup_vs_plain_fp64": (round(plain["ms_per_solve"] / ms, 3) if plain else None),
+        "error_vs_fp64": err_vs_fp64,
     }
     print(json.dumps(out), flush=True)
 

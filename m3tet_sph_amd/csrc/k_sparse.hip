@@ -56,20 +56,38 @@ __device__ inline auto ldv(const MT *p) {
 // y = K x   (optionally fused: r = r0 - K x; d_out = x + w Dinv r; dot partial)
 // mode bits: 1 write y, 2 compute r = r0 - y and write r_out, 4 write d_out,
 // 8 accumulate partial dot(x, y) into part[blockIdx]
-template <class MT, class VT, int MODE>
+// TPR threads cooperate on one row (TPR=1 on the finest level; 4 on small
+// levels so that a 43k-row level still fills the 256 CUs); the k loop is
+// unrolled by two so that two independent gathers are in flight per thread.
+template <class MT, class VT, int MODE, int TPR>
 __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const VT *__restrict__ x,
                                                 const VT *__restrict__ r0, const VT *__restrict__ dinv,
                                                 VT w, VT *__restrict__ y, VT *__restrict__ r_out,
                                                 VT *__restrict__ d_out, double *__restrict__ part) {
   using A = typename Acc<MT, VT>::T;
-  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t gt = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gt / TPR;
+  const int sub = (int)(gt % TPR);
   A y0 = 0, y1 = 0, y2 = 0;
   if (row < nrow) {
     const int64_t base = K.slice_ptr[row >> 6] + (row & 63);
     const int len = K.rowlen[row];
     const int64_t ns = K.nslot;
     const MT *v = K.val;
-    for (int k = 0; k < len; ++k) {
+    int k = sub;
+    for (; k + TPR < len; k += 2 * TPR) {
+      const int64_t p = base + (int64_t)k * 64, q = p + 64 * TPR;
+      const int64_t cp = 3 * (int64_t)K.col[p], cq = 3 * (int64_t)K.col[q];
+      const A x0 = (A)x[cp], x1 = (A)x[cp + 1], x2 = (A)x[cp + 2];
+      const A z0 = (A)x[cq], z1 = (A)x[cq + 1], z2 = (A)x[cq + 2];
+      y0 += (A)ldv(v + p) * x0 + (A)ldv(v + ns + p) * x1 + (A)ldv(v + 2 * ns + p) * x2;
+      y1 += (A)ldv(v + 3 * ns + p) * x0 + (A)ldv(v + 4 * ns + p) * x1 + (A)ldv(v + 5 * ns + p) * x2;
+      y2 += (A)ldv(v + 6 * ns + p) * x0 + (A)ldv(v + 7 * ns + p) * x1 + (A)ldv(v + 8 * ns + p) * x2;
+      y0 += (A)ldv(v + q) * z0 + (A)ldv(v + ns + q) * z1 + (A)ldv(v + 2 * ns + q) * z2;
+      y1 += (A)ldv(v + 3 * ns + q) * z0 + (A)ldv(v + 4 * ns + q) * z1 + (A)ldv(v + 5 * ns + q) * z2;
+      y2 += (A)ldv(v + 6 * ns + q) * z0 + (A)ldv(v + 7 * ns + q) * z1 + (A)ldv(v + 8 * ns + q) * z2;
+    }
+    if (k < len) {
       const int64_t p = base + (int64_t)k * 64;
       const int64_t cc = 3 * (int64_t)K.col[p];
       const A x0 = (A)x[cc], x1 = (A)x[cc + 1], x2 = (A)x[cc + 2];
@@ -77,6 +95,15 @@ __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const
       y1 += (A)ldv(v + 3 * ns + p) * x0 + (A)ldv(v + 4 * ns + p) * x1 + (A)ldv(v + 5 * ns + p) * x2;
       y2 += (A)ldv(v + 6 * ns + p) * x0 + (A)ldv(v + 7 * ns + p) * x1 + (A)ldv(v + 8 * ns + p) * x2;
     }
+  }
+  if (TPR > 1) {
+    for (int off = TPR / 2; off > 0; off >>= 1) {
+      y0 += __shfl_xor(y0, off, TPR);
+      y1 += __shfl_xor(y1, off, TPR);
+      y2 += __shfl_xor(y2, off, TPR);
+    }
+  }
+  if (row < nrow && sub == 0) {
     const int64_t o = 3 * row;
     if (MODE & 1) { y[o] = (VT)y0; y[o + 1] = (VT)y1; y[o + 2] = (VT)y2; }
     if (MODE & 2) {
@@ -91,7 +118,7 @@ __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const
   }
   if (MODE & 8) {
     double s = 0;
-    if (row < nrow) {
+    if (row < nrow && sub == 0) {
       const int64_t o = 3 * row;
       s = (double)x[o] * (double)y0 + (double)x[o + 1] * (double)y1 + (double)x[o + 2] * (double)y2;
     }
@@ -406,9 +433,45 @@ inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) /
 
 namespace m3 {
 
+// threads per row for a level of n rows: one thread per row when the level
+// alone fills the GPU, four otherwise (M3_TPR_FINE / M3_TPR_COARSE override)
+static int tpr_for(int64_t n) {
+  static int fine = -1, coarse = -1;
+  if (fine < 0) {
+    const char *e1 = getenv("M3_TPR_FINE"), *e2 = getenv("M3_TPR_COARSE");
+    fine = e1 ? atoi(e1) : 1;
+    coarse = e2 ? atoi(e2) : 4;
+  }
+  return n >= 131072 ? fine : coarse;
+}
+
+template <class MT, class VT, int MODE>
+static void launch_smooth(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
+                          double w, VT *y, VT *r_out, VT *d_out, double *part) {
+  const int t = tpr_for(n);
+  if (MODE & 8) {  // partial sums are indexed per block of 256 rows
+    hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
+                       y, r_out, d_out, part);
+    return;
+  }
+  switch (t) {
+    case 2:
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 2>), dim3(nblk(2 * n)), dim3(256), 0, s, K, n, x, r0, dinv,
+                         (VT)w, y, r_out, d_out, part);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 4>), dim3(nblk(4 * n)), dim3(256), 0, s, K, n, x, r0, dinv,
+                         (VT)w, y, r_out, d_out, part);
+      break;
+    default:
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
+                         (VT)w, y, r_out, d_out, part);
+  }
+}
+
 void spmv_K(hipStream_t s, LevelDev &lv, const double *x, double *y) {
-  hipLaunchKernelGGL((k_smooth<double, double, 1>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.K.view(),
-                     lv.nnod, x, nullptr, nullptr, 0.0, y, nullptr, nullptr, nullptr);
+  launch_smooth<double, double, 1>(s, lv.K.view(), lv.nnod, x, nullptr, nullptr, 0.0, y, nullptr, nullptr,
+                                   nullptr);
 }
 
 void spmv_G(hipStream_t s, Sell &G, const double *q, double *y, const double *mask) {
@@ -460,11 +523,9 @@ template <class MT, class VT>
 static void sweep(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
                   double w, VT *r_out, VT *d_out) {
   if (d_out)
-    hipLaunchKernelGGL((k_smooth<MT, VT, 6>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
-                       (VT *)nullptr, r_out, d_out, (double *)nullptr);
+    launch_smooth<MT, VT, 6>(s, K, n, x, r0, dinv, w, (VT *)nullptr, r_out, d_out, (double *)nullptr);
   else
-    hipLaunchKernelGGL((k_smooth<MT, VT, 2>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
-                       (VT *)nullptr, r_out, (VT *)nullptr, (double *)nullptr);
+    launch_smooth<MT, VT, 2>(s, K, n, x, r0, dinv, w, (VT *)nullptr, r_out, (VT *)nullptr, (double *)nullptr);
 }
 
 template <class VT, class Pick>
@@ -656,9 +717,8 @@ int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, in
   const int nb = red_blocks(n3);
   // r = b - K x  (masked: fixed entries of b are zeroed)
   hipLaunchKernelGGL(k_scale_mask, dim3(nblk(n3)), dim3(256), 0, s, n3, b, f.mask.p, c->cg_b.p);
-  hipLaunchKernelGGL((k_smooth<double, double, 2>), dim3(nblk(f.nnod)), dim3(256), 0, s, f.K.view(), f.nnod,
-                     x, c->cg_b.p, (const double *)nullptr, 0.0, (double *)nullptr, r, (double *)nullptr,
-                     (double *)nullptr);
+  launch_smooth<double, double, 2>(s, f.K.view(), f.nnod, x, c->cg_b.p, (const double *)nullptr, 0.0,
+                                   (double *)nullptr, r, (double *)nullptr, (double *)nullptr);
   hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RB), 0, s, n3, r, r, c->partial.p);
   hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p + 5);
   HIPCHECK(hipMemcpyAsync(c->h_scal + 5, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -675,9 +735,9 @@ int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, in
                        c->partial.p, nb);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p, it == 1 ? 1 : 0);
     hipLaunchKernelGGL(k_cg_dir, dim3(nblk(n3)), dim3(256), 0, s, n3, d, q, c->scal.p);
-    hipLaunchKernelGGL((k_smooth<double, double, 9>), dim3(gK), dim3(256), 0, s, f.K.view(), f.nnod, q,
-                       (const double *)nullptr, (const double *)nullptr, 0.0, Kq, (double *)nullptr,
-                       (double *)nullptr, c->partial.p);
+    launch_smooth<double, double, 9>(s, f.K.view(), f.nnod, q, (const double *)nullptr,
+                                     (const double *)nullptr, 0.0, Kq, (double *)nullptr, (double *)nullptr,
+                                     c->partial.p);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(RB), 0, s, c->partial.p, (int)gK, c->scal.p);
     hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(RB), 0, s, n3, x, q, r, rl, Kq, c->scal.p, c->partial.p);
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p + 5);

@@ -257,3 +257,19 @@ def test_c1_full_solve_reaches_tolerance(c1ctx):
     assert rU / fU < 1e-6
     assert rP / info["norm_rP0"] < 1e-6
     assert np.allclose(U[inp["ifix"]], inp["vfix"], atol=1e-14)
+
+
+def test_c1_mixed_precision_reaches_tolerance(c1ctx):
+    """FP32 V-cycle with BF16 coarse-level values inside the FP64 outer CG:
+    same fixed-denominator tolerance, error vs the FP64 solve stated."""
+    from m3tet_sph_amd import api
+    m, inp, ctx = c1ctx
+    U64, P64, i64 = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], rtol_Pat=1e-6, rtol_U_final=5e-7)
+    U, P, info = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], rtol_Pat=1e-6, rtol_U_final=5e-7,
+                            precision="mixed")
+    rU, fU, rP = api.residual(ctx, U, P)
+    eu, ep = relerr(U, U64), relerr(P, P64)
+    log("mixed", info, rU / fU, rP / i64["norm_rP0"], "err vs fp64", eu, ep)
+    assert rU / fU < 1e-6
+    assert rP / i64["norm_rP0"] < 1e-6
+    assert eu < 1e-4 and ep < 1e-4
