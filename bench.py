@@ -11,9 +11,24 @@ the same denominator for every form).  Inputs are seeded synthetic fields
 (temperature with spherical-harmonic perturbation, viscosity contrast 1e4,
 6 Voronoi plates at the surface, free slip at the CMB).
 
+Forms:
+  learned (default): the headline metric.  Input is a held-out time step of
+    the seeded c1 convection trajectory (vertex temperatures of the step and
+    of the step before, shipped in m3tet_sph_amd/data).  Untimed: the FP64
+    solve of the previous step gives (U_prev, P_prev); a cold FP64 solve of
+    the step gives |r_P^cold|.  Timed, each as K solves after W warm-ups:
+      plain FP64 MGPCG from a cold start (the reference solver as configured
+        for a single solve: P = 0, U from the BCs);
+      FP64 from the previous step (the reference's time loop warm start);
+      mixed precision + learned warm start: the network's prediction from
+        (T, ln eta, U_prev, P_prev) followed by the mixed-precision solve;
+    all stopping at the same fixed residual.  ``value`` is the mixed+learned
+    rate; the others are reported beside it.
+  fp64 / mixed: cold solves of the seed-0 temperature field.
+
 Prints one JSON line.  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
-                  [--form fp64|mixed] [--no-cpu]
+                  [--form learned|fp64|mixed] [--no-cpu]
 """
 import argparse
 import json
@@ -91,13 +106,35 @@ def cpu_baseline(ctx, mesh, inp, n_cg, seed=0):
                 t_solve_s=t_solve, t_assembly_s=t_asm, t_cg_iter_s=t_it, t_coarse_factor_s=t_fac)
 
 
+def roofline(ctx, mesh, api, run_once, form):
+    """Average duration of the fine-level fused sweep over one solve (HIP
+    events on the solver's stream) and its algorithmic bytes."""
+    api.profile(ctx, True)
+    run_once()
+    pr = api.profile_read(ctx)
+    api.profile(ctx, False)
+    nnzb, nnod = ctx.stats()["nnzb"][0], mesh.nnod
+    if form == "fp64":
+        nsw, tsw, bpl = pr["n_fp64"], pr["t_fp64"], sweep_bytes(nnzb, nnod, 8, 8)
+    else:
+        nsw, tsw, bpl = pr["n_mixed"], pr["t_mixed"], sweep_bytes(nnzb, nnod, 4, 4)
+    avg = tsw / max(nsw, 1)
+    achieved = bpl / avg / 1e9
+    log(f"sweep: {nsw} launches avg {avg * 1e6:.1f} us, {bpl / 1e6:.1f} MB -> {achieved:.0f} GB/s")
+    kern = "k_smooth<double,double,6,1>" if form == "fp64" else "k_smooth<float,float,6,1>"
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel=f"{kern}: fine-level fused Jacobi sweep", launches=nsw,
+                avg_us=round(avg * 1e6, 2), bytes_per_launch=int(bpl))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--form", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--form", default="learned", choices=["learned", "fp64", "mixed"])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -113,67 +150,95 @@ def main():
     inp = F.make_inputs(mesh, visc_contrast=1e4, Ra=1e5, seed_T=0, seed_plates=1)
     ctx = api.StokesContext(mesh)
     ctx.set_bc(inp["ifix"], inp["vfix"])
+    phys = inp["phys"]
     log(f"setup {time.perf_counter() - t0:.1f}s config={args.config} nel={mesh.nel} "
         f"nU={3 * mesh.nnod} nP={mesh.nVnod} stats={ctx.stats()}")
 
-    def solve(form):
-        return ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], P0=None, rtol_Pat=args.tol,
-                          rtol_U_final=0.5 * args.tol, precision=form)
+    net = None
+    workload = "seed-0 temperature field, cold start"
+    if args.form == "learned":
+        from m3tet_sph_amd import learned as L
+        data = os.path.join(HERE, "m3tet_sph_amd", "data")
+        wpath = os.path.join(data, f"warm_start_{args.config}.npz")
+        spath = os.path.join(data, f"{args.config}_heldout_states.npz")
+        if not (os.path.exists(wpath) and os.path.exists(spath)):
+            raise SystemExit(f"bench.py: trained network or held-out states missing ({wpath}, {spath}); "
+                             f"run python -m m3tet_sph_amd.train_warm_start first")
+        net = L.WarmStartNet(mesh)
+        net.load(wpath)
+        z = np.load(spath, allow_pickle=False)
+        step = int(z["steps"][-1])
+        Tv = z["T"][-1].astype(np.float64)
+        Tp = z["T_prev"][-1].astype(np.float64)
+        Vp, Dp = F.material(L.nodal_temperature(mesh, Tp, net.edge_parent), 1e4)
+        U_prev, P_prev, _ = ctx.flow3d(Vp, Dp, phys, rtol_Pat=0.1 * args.tol, rtol_U_final=0.1 * args.tol)
+        Visc, Dens = F.material(L.nodal_temperature(mesh, Tv, net.edge_parent), 1e4)
+        workload = f"held-out step {step} of the seeded convection trajectory (seed 2)"
+    else:
+        Visc, Dens = inp["Visc"], inp["Dens"]
+
+    def cold(form):
+        return ctx.flow3d(Visc, Dens, phys, P0=None, rtol_Pat=args.tol, rtol_U_final=0.5 * args.tol,
+                          precision=form)
 
     # fixed denominator: pressure residual of the velocity solve at P = 0
-    U, P, info = solve("fp64")
+    U, P, info = cold("fp64")
     rP_cold = info["norm_rP0"]
+    tol_abs = args.tol * rP_cold
+
+    def warm(form, U0, P0):
+        return ctx.flow3d(Visc, Dens, phys, P0=P0, U0=U0, precision=form, rtol_U_final=0.5 * args.tol,
+                          tol_Pat_abs=tol_abs)
+
+    def learned(form):
+        U0, P0 = net.warm_start(Tv, Tp, Visc, U_prev, P_prev)
+        return warm(form, U0, P0)
 
     def rel_res(U, P):
         rU, fU, rP = api.residual(ctx, U, P)
         return max(rU / fU, rP / rP_cold), rU / fU, rP / rP_cold
 
-    def timed(form):
+    def timed(fn):
         for _ in range(args.warmup):
-            solve(form)
+            fn()
         t0 = time.perf_counter()
         infos = []
         for _ in range(args.steps):
-            U, P, info = solve(form)
+            U, P, info = fn()
             infos.append(info)
-        return (time.perf_counter() - t0), U, P, infos
+        dt = time.perf_counter() - t0
+        rr = rel_res(U, P)
+        res = dict(solves_per_s=round(args.steps / dt, 4), ms_per_solve=round(1e3 * dt / args.steps, 2),
+                   rel_res=float(f"{rr[0]:.3e}"), nitP=infos[-1]["nitP"], nitU=infos[-1]["nitU"],
+                   nitZ=infos[-1]["nitZ"])
+        return res, U, P, infos
 
-    plain = None
-    if args.form != "fp64":
-        dtp, Up, Pp, infp = timed("fp64")
-        plain = dict(solves_per_s=round(args.steps / dtp, 4), ms_per_solve=round(1e3 * dtp / args.steps, 2),
-                     rel_res=float(f"{rel_res(Up, Pp)[0]:.3e}"),
-                     nitZ=infp[-1]["nitZ"], nitU=infp[-1]["nitU"], nitP=infp[-1]["nitP"])
-        log(f"plain fp64: {plain}")
-    dt, U, P, infos = timed(args.form)
-    rr, rru, rrp = rel_res(U, P)
-    ms = 1e3 * dt / args.steps
-    err_vs_fp64 = None
-    if plain is not None:
-        err_vs_fp64 = dict(U=float(f"{np.linalg.norm(U - Up) / np.linalg.norm(Up):.3e}"),
-                           P=float(f"{np.linalg.norm(P - Pp) / np.linalg.norm(Pp):.3e}"))
-    n_cg = int(np.mean([i["nitU"] + i["nitZ"] for i in infos]))
-    log(f"{args.steps} solves in {dt:.3f}s -> {ms:.1f} ms/solve; rel_res={rr:.2e} "
-        f"(U {rru:.2e}, P {rrp:.2e}); last info {infos[-1]}")
-
-    # roofline of the dominant kernel: fine-level fused Jacobi sweep
-    api.profile(ctx, True)
-    solve(args.form)
-    pr = api.profile_read(ctx)
-    api.profile(ctx, False)
-    st = ctx.stats()
-    nnzb, nnod = st["nnzb"][0], mesh.nnod
-    if args.form == "fp64":
-        nsw, tsw, bpl = pr["n_fp64"], pr["t_fp64"], sweep_bytes(nnzb, nnod, 8, 8)
+    legs = {}
+    plain, Up, Pp, plain_infos = timed(lambda: cold("fp64"))
+    log(f"plain fp64 (cold): {plain}")
+    if args.form == "learned":
+        legs["fp64_prev_step"], _, _, _ = timed(lambda: warm("fp64", U_prev, P_prev))
+        legs["mixed_prev_step"], _, _, _ = timed(lambda: warm("mixed", U_prev, P_prev))
+        legs["fp64_learned"], _, _, _ = timed(lambda: learned("fp64"))
+        for k, v in legs.items():
+            log(f"{k}: {v}")
+        main_fn, vform = (lambda: learned("mixed")), "mixed"
+    elif args.form == "mixed":
+        main_fn, vform = (lambda: cold("mixed")), "mixed"
     else:
-        nsw, tsw, bpl = pr["n_mixed"], pr["t_mixed"], sweep_bytes(nnzb, nnod, 4, 4)
-    avg = tsw / max(nsw, 1)
-    achieved = bpl / avg / 1e9
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_smooth fine-level fused Jacobi sweep", launches=nsw,
-                avg_us=round(avg * 1e6, 2), bytes_per_launch=int(bpl))
-    log(f"sweep: {nsw} launches avg {avg * 1e6:.1f} us, {bpl / 1e6:.1f} MB -> {achieved:.0f} GB/s")
+        main_fn, vform = (lambda: cold("fp64")), "fp64"
+    if args.form == "fp64":
+        res, U, P, infos = plain, Up, Pp, plain_infos
+    else:
+        res, U, P, infos = timed(main_fn)
+    ms = res["ms_per_solve"]
+    rr, rru, rrp = rel_res(U, P)
+    err_vs_fp64 = dict(U=float(f"{np.linalg.norm(U - Up) / np.linalg.norm(Up):.3e}"),
+                       P=float(f"{np.linalg.norm(P - Pp) / np.linalg.norm(Pp):.3e}"))
+    n_cg = int(np.mean([i["nitU"] + i["nitZ"] for i in infos]))
+    log(f"{args.form}: {res}; rel_res={rr:.2e} (U {rru:.2e}, P {rrp:.2e}); last info {infos[-1]}")
+
+    roof = roofline(ctx, mesh, api, main_fn, vform)
 
     cpu = None
     if not args.no_cpu:
@@ -193,13 +258,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f64 outer, f32/bf16 V-cycle",
+        "dtype": "f64" if vform == "fp64" else "f64 outer, f32/bf16 V-cycle",
         "data": "synthetic",
         "config": {"workload": f"{args.config}: spherical-shell Stokes, {mesh.nel} P2 tets, "
                                f"{3 * mesh.nnod} velocity dofs, {mesh.nVnod} pressure dofs, "
-                               f"{mesh.nmg}-level MG, viscosity contrast 1e4",
-                   "form": args.form, "rel_res_target": args.tol, "seeds": [0, 1, 2],
-                   "Ra": 1e5},
+                               f"{mesh.nmg}-level MG, viscosity contrast 1e4; {workload}",
+                   "form": {"learned": "mixed+learned", "fp64": "plain fp64", "mixed": "mixed"}[args.form],
+                   "rel_res_target": args.tol, "seeds": [0, 1, 2], "Ra": 1e5},
         "rel_res": float(f"{rr:.3e}"),
         "iterations": {"nitP": last["nitP"], "nitU": last["nitU"], "nitZ": last["nitZ"]},
         "phases_ms": {k: round(1e3 * last[k], 2) for k in
@@ -207,9 +272,12 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_fp64": plain,
-        "speedup_vs_plain_fp64": (round(plain["ms_per_solve"] / ms, 3) if plain else None),
+        "speedup_vs_plain_fp64": round(plain["ms_per_solve"] / ms, 3),
         "error_vs_fp64": err_vs_fp64,
     }
+    if legs:
+        out["legs"] = legs
+        out["speedup_vs_fp64_prev_step"] = round(legs["fp64_prev_step"]["ms_per_solve"] / ms, 3)
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,113 @@
+/*
+ * m3gnn.h -- C ABI of the learned warm start of the Stokes solve.
+ *
+ * The reference has no counterpart: flow3d_patera_mg_sph_p.m:204-231 starts
+ * the Patera iteration from the previous time step's velocity and pressure
+ * (VAR.Ux/Uy/Uz, VAR.P).  These entry points predict a better starting pair
+ * (U0, P0) for the same call from the new temperature, the log-viscosity and
+ * the previous (U, P), with a graph network on the P1 vertex mesh.  The
+ * solver (m3_flow3d_patera_sph with warm_U = 1) then iterates the prediction
+ * to the same residual tolerance as any other starting point, so the network
+ * never decides the accuracy of the result.
+ *
+ * Network (all on the vertex graph, n = number of P1 vertices):
+ *   x    = (features - in_mu) / in_sigma                     n x 8 (FP32)
+ *   h0   = relu(x We + be)                                    n x 64 (BF16)
+ *   for l = 1..3:
+ *     m_l  = mean over graph neighbours of h_{l-1}            n x 64 (BF16)
+ *     z_l  = relu([h_{l-1}, m_l] W_l + b_l)                   n x 64 (BF16)
+ *     h_l  = h_{l-1} + z_l                                    n x 64 (BF16)
+ *   y    = h_3 Wd + bd                                        n x 4  (FP32)
+ * Weights and all arithmetic are FP32; activations are stored in BF16.
+ * Features per vertex: T, ln(viscosity), previous (u_theta, u_phi, u_r)/s_u,
+ * previous p/s_p, (r - r_in)/(r_out - r_in), T - T_prev, with s_u the RMS
+ * vertex speed and s_p the RMS vertex pressure of the previous state.  The
+ * output is the normalised change: U0 = U_prev + y[0:3] out_sigma[0:3] s_u
+ * at vertices (edge nodes take the mean of their two vertices' changes),
+ * P0 = P_prev + y[3] out_sigma[3] s_p.
+ *
+ * Parameter vector (FP32, M3_GNN_NPARAM values): We (8x64 row-major, input
+ * index first), be (64), then per layer W_l (128x64) and b_l (64), then Wd
+ * (64x4) and bd (4).  Normalisation vector (M3_GNN_NNORM values): in_mu (8),
+ * in_sigma (8), out_sigma (4).
+ *
+ * Conventions as in m3stokes.h: host pointers, synchronous calls, 0 or an
+ * M3_E* code, message from m3_gnn_last_error.
+ */
+#ifndef M3GNN_H
+#define M3GNN_H
+
+#include <stdint.h>
+
+#include "m3stokes.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3_GNN_NF 8
+#define M3_GNN_WIDTH 64
+#define M3_GNN_NOUT 4
+#define M3_GNN_LAYERS 3
+#define M3_GNN_NPARAM (8 * 64 + 64 + 3 * (128 * 64 + 64) + 64 * 4 + 4)
+#define M3_GNN_NNORM (8 + 8 + 4)
+
+typedef struct m3_gnn m3_gnn;
+
+/* Vertex graph of the finest P1 mesh (the edges of the 4-node tetrahedra
+ * EL2NOD{1}(1:4,:)) and the edge nodes of the P2 mesh. */
+typedef struct {
+  int64_t nvert;              /* P1 vertices = pressure dofs; they are P2 nodes 0..nvert-1 */
+  int64_t nnod;               /* P2 nodes */
+  const int64_t *adj_ptr;     /* nvert+1, CSR of the symmetric neighbour lists */
+  const int32_t *adj_col;     /* 0-based neighbours, no self loops */
+  const int32_t *edge_parent; /* 2 x (nnod - nvert): the two vertices (0-based) of
+                                 edge node nvert+k at [2k], [2k+1] */
+  const double *radius;       /* nvert vertex radii */
+  double r_in, r_out;
+} m3_graph_desc;
+
+int m3_gnn_create(m3_gnn **g, int hip_device, const m3_graph_desc *graph);
+void m3_gnn_destroy(m3_gnn *g);
+const char *m3_gnn_last_error(const m3_gnn *g);
+
+/* norm may be NULL to keep the normalisation fitted by m3_gnn_fit_norm;
+ * either output of m3_gnn_get_params may be NULL. */
+int m3_gnn_set_params(m3_gnn *g, const float *params, const float *norm);
+int m3_gnn_get_params(m3_gnn *g, float *params, float *norm);
+
+/* Features (n x 8, FP32) and the scales (s_u, s_p) of one state. */
+int m3_gnn_features(m3_gnn *g, const double *T, const double *T_prev, const double *Visc,
+                    const double *U_prev, const double *P_prev, float *X, double *scales);
+
+/* Training set: sample idx < nsample_max holds the features of
+ * (T, T_prev, Visc, U_prev, P_prev) and the target change of the converged
+ * (U_next, P_next).  U arrays are 3*nnod in the rotated (theta, phi, r) frame,
+ * P arrays nvert, T nvert, Visc nnod (vertex entries are read). */
+int m3_gnn_reserve(m3_gnn *g, int32_t nsample_max);
+int m3_gnn_add_sample(m3_gnn *g, int32_t idx, const double *T, const double *T_prev, const double *Visc,
+                      const double *U_prev, const double *P_prev, const double *U_next,
+                      const double *P_next);
+/* in_mu, in_sigma, out_sigma from samples [0, nsample). */
+int m3_gnn_fit_norm(m3_gnn *g, int32_t nsample);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) steps, one per listed sample;
+ * loss[k] receives the mean squared error of step k (before its update). */
+int m3_gnn_train(m3_gnn *g, const int32_t *samples, int32_t nsteps, float lr, float *loss);
+/* Mean squared error of the listed samples under the current parameters. */
+int m3_gnn_eval(m3_gnn *g, const int32_t *samples, int32_t n, float *loss);
+
+/* One forward and backward pass on caller features X (n x 8) and raw targets
+ * Y (n x 4, divided by out_sigma inside): prediction (n x 4, normalised),
+ * loss and the gradient of the loss with respect to every parameter. */
+int m3_gnn_forward_backward(m3_gnn *g, const float *X, const float *Y, float *pred, float *loss,
+                            float *grad);
+
+/* Warm start for the next solve: U0 (3*nnod) and P0 (nvert). */
+int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const double *Visc,
+                      const double *U_prev, const double *P_prev, double *U0, double *P0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

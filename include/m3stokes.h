@@ -118,6 +118,11 @@ typedef struct {
                           |r_U| <= rtol_U_final |f_U| (fixed-denominator residual),
                           with up to itmax_U_final iterations */
   int32_t itmax_U_final;
+  double tol_Pat_abs;  /* 0: reference behaviour (tol_Pat = rtol_Pat * initial
+                          pressure residual, flow3d_patera_mg_sph_p.m:436-437).
+                          > 0: absolute RMS pressure-residual tolerance, so that
+                          runs from different starting points stop at the same
+                          residual */
 } m3_patera_opts;
 
 typedef struct {
@@ -199,6 +204,20 @@ int m3_profile_read(m3_ctx *ctx, double *out, int32_t nout);
  * rowptr (nnod+1), col (nnzb), val (9*nnzb).  Pass NULL pointers to query
  * nothing but sizes via m3_stats.  No reference counterpart (diagnostics). */
 int m3_get_operator(m3_ctx *ctx, int32_t level, int64_t *rowptr, int32_t *col, double *val);
+
+/* Semi-Lagrangian temperature update on the P1 vertices over one time step,
+ * the step after the Stokes solve in the time loop
+ * (mfiles_SPH/advect3d_slm_p_sph.m:1, backtrack_PC2 :165-243): each vertex
+ * is traced back over dt with a predictor-corrector step along great circles
+ * (the tangential velocity moves a point along the sphere of its radius, the
+ * radial velocity changes the radius), departure points outside the shell are
+ * shifted onto the nearest boundary (find_pts_outside_domain_sph), and T is
+ * interpolated linearly in the straight P1 tetrahedron that holds the point.
+ * T and T_out: nVnod vertex temperatures; U: 3*nnod velocity in the rotated
+ * (theta, phi, r) frame as returned by m3_flow3d_patera_sph (vertex nodes are
+ * read); vertices on the CMB / surface get T_cmb / T_surf. */
+int m3_sl_advect(m3_ctx *ctx, const double *T, const double *U, double dt, double T_cmb, double T_surf,
+                 double *T_out);
 
 #ifdef __cplusplus
 }

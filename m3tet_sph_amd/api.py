@@ -66,7 +66,7 @@ class PateraOpts(C.Structure):
     _fields_ = [("rtol_Pat", C.c_double), ("rtol_U", C.c_double), ("itmax_U", C.c_int32),
                 ("itmin_Pat", C.c_int32), ("itmax_Pat", C.c_int32), ("pc_Pat", C.c_int32),
                 ("wght_MM", C.c_double), ("precision", C.c_int32), ("warm_U", C.c_int32),
-                ("rtol_U_final", C.c_double), ("itmax_U_final", C.c_int32)]
+                ("rtol_U_final", C.c_double), ("itmax_U_final", C.c_int32), ("tol_Pat_abs", C.c_double)]
 
 
 class Info(C.Structure):
@@ -99,9 +99,11 @@ LIB.m3_flow3d_patera_sph.argtypes = [C.c_void_p, D, D, C.POINTER(Phys), D, D,
 LIB.m3_apply.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D, D]
 LIB.m3_get_vector.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D]
 LIB.m3_stats.argtypes = [C.c_void_p, I64, C.c_int32]
+LIB.m3_sl_advect.argtypes = [C.c_void_p, D, D, C.c_double, C.c_double, C.c_double, D]
 for _f in ("m3_create", "m3_set_mesh", "m3_set_bc", "m3_assemble", "m3_element_matrices",
            "m3_restrict_stiffmat", "m3_cholesky_factorization", "m3_cholesky_solve", "m3_mgpcg",
-           "m3_patera", "m3_flow3d_patera_sph", "m3_apply", "m3_get_vector", "m3_stats"):
+           "m3_patera", "m3_flow3d_patera_sph", "m3_apply", "m3_get_vector", "m3_stats",
+           "m3_sl_advect"):
     getattr(LIB, _f).restype = C.c_int
 
 
@@ -122,15 +124,18 @@ def phys_struct(phys, nip=14, type_Dmat="23rd"):
 
 
 def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=120,
-                wght_MM=1.0, precision="fp64", warm_U=False, rtol_U_final=0.0, itmax_U_final=60):
+                wght_MM=1.0, precision="fp64", warm_U=False, rtol_U_final=0.0, itmax_U_final=60,
+                tol_Pat_abs=0.0):
     """Defaults of flow3d_patera_mg_sph_p.m:70-98 (rtol_U_final=0 keeps the
-    reference's final velocity tolerance)."""
+    reference's final velocity tolerance, tol_Pat_abs=0 its relative
+    pressure tolerance)."""
     o = PateraOpts()
     o.rtol_Pat, o.rtol_U, o.itmax_U = rtol_Pat, rtol_U, itmax_U
     o.itmin_Pat, o.itmax_Pat, o.pc_Pat, o.wght_MM = itmin_Pat, itmax_Pat, 3, wght_MM
     o.precision = PREC[precision]
     o.warm_U = int(warm_U)
     o.rtol_U_final, o.itmax_U_final = rtol_U_final, itmax_U_final
+    o.tol_Pat_abs = tol_Pat_abs
     return o
 
 
@@ -271,6 +276,15 @@ class StokesContext:
                                              C.byref(phys_struct(phys, type_Dmat=type_Dmat)),
                                              _p(U), _p(P), C.byref(o), C.byref(info)))
         return U, P, info.as_dict()
+
+    def sl_advect(self, T, U, dt, T_cmb=1.0, T_surf=0.0):
+        """Semi-Lagrangian update of the vertex temperatures T (nVnod) over dt
+        with the rotated-frame velocity U (3*nnod), advect3d_slm_p_sph.m:1."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        out = np.zeros_like(T)
+        self._check(LIB.m3_sl_advect(self._h, _p(T), _p(U), float(dt), float(T_cmb), float(T_surf), _p(out)))
+        return out
 
     def apply(self, which, level, x):
         m = self.mesh

@@ -1,0 +1,234 @@
+// Semi-Lagrangian temperature advection on the vertices of the finest P1
+// mesh: the time-loop step that follows the Stokes solve
+// (mfiles_SPH/advect3d_slm_p_sph.m:1-140, predictor-corrector backtracking
+// backtrack_PC2 :165-243 with the spherical projections of
+// predictor_step_with_projections / corrector_step_with_projections
+// :298-440, boundary shift find_pts_outside_domain_sph, linear interpolation
+// in the located tetrahedron).
+//
+// One thread per vertex.  Back-tracking moves a point along the great circle
+// of its tangential velocity by the arc dt*|u_t|/r and changes its radius by
+// -dt*u_r; the predictor uses the vertex velocity over dt/2, the corrector the
+// velocity interpolated at the predicted point, projected on the tangent
+// plane of the vertex, over dt.  Points are located among the tetrahedra
+// around the vertex (then around the vertices of the best candidate); a
+// point outside every candidate takes the candidate whose smallest
+// barycentric coordinate is largest, with negative coordinates clipped.
+#include "m3_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+struct AdvArgs {
+  int64_t nV;
+  const double *X;        // 3*nnod node coordinates (vertices first)
+  const double *frames;   // 9*nnod rotation frames
+  const double *U;        // 3*nnod rotated velocity
+  const double *T;
+  const int64_t *star_ptr;
+  const int32_t *star_tet, *tet_v;
+  const double *tet_geo;
+  const double *uc;       // 3*nV Cartesian vertex velocity
+  double r_in, r_out, dt, T_cmb, T_surf;
+  double *T_out;
+};
+
+__global__ void k_vel_cart(int64_t nV, const double *frames, const double *U, double *uc) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const double *R = frames + 9 * v;
+  const double a = U[3 * v], b = U[3 * v + 1], c = U[3 * v + 2];
+  for (int i = 0; i < 3; ++i) uc[3 * v + i] = R[3 * i] * a + R[3 * i + 1] * b + R[3 * i + 2] * c;
+}
+
+__device__ inline void bary(const double *g, const double p[3], double l[4]) {
+  const double d0 = p[0] - g[9], d1 = p[1] - g[10], d2 = p[2] - g[11];
+  l[0] = g[0] * d0 + g[1] * d1 + g[2] * d2;
+  l[1] = g[3] * d0 + g[4] * d1 + g[5] * d2;
+  l[2] = g[6] * d0 + g[7] * d1 + g[8] * d2;
+  l[3] = 1.0 - l[0] - l[1] - l[2];
+}
+
+__device__ inline void scan_star(const AdvArgs &a, int64_t v, const double p[3], double &best, int32_t &bt,
+                                 double bl[4]) {
+  for (int64_t k = a.star_ptr[v]; k < a.star_ptr[v + 1]; ++k) {
+    const int32_t t = a.star_tet[k];
+    double l[4];
+    bary(a.tet_geo + 12 * (int64_t)t, p, l);
+    const double mn = fmin(fmin(l[0], l[1]), fmin(l[2], l[3]));
+    if (mn > best) {
+      best = mn;
+      bt = t;
+      for (int i = 0; i < 4; ++i) bl[i] = l[i];
+    }
+  }
+}
+
+// tetrahedron and clipped barycentric coordinates of p, searched from vertex v
+__device__ inline int32_t locate(const AdvArgs &a, int64_t v, const double p[3], double l[4]) {
+  double best = -1e300;
+  int32_t bt = -1;
+  scan_star(a, v, p, best, bt, l);
+  if (best < -1e-12 && bt >= 0) {
+    int32_t w[4];
+    for (int i = 0; i < 4; ++i) w[i] = a.tet_v[4 * (int64_t)bt + i];
+    for (int i = 0; i < 4; ++i)
+      if (w[i] != v) scan_star(a, w[i], p, best, bt, l);
+  }
+  if (bt < 0) {  // only for non-finite input: any tetrahedron of the star
+    bt = a.star_tet[a.star_ptr[v]];
+    for (int i = 0; i < 4; ++i) l[i] = 0.25;
+    return bt;
+  }
+  if (best < 0) {
+    double s = 0;
+    for (int i = 0; i < 4; ++i) {
+      l[i] = fmax(l[i], 0.0);
+      s += l[i];
+    }
+    for (int i = 0; i < 4; ++i) l[i] /= s;
+  }
+  return bt;
+}
+
+// x = r*xh traced back over tau: arc tau*|u_t|/r_ang along the tangential
+// part of u at xh, radius change -tau*(u . rh)
+__device__ inline void gc_back(const double xh[3], double r, const double u[3], double tau, double r_ang,
+                               const double rh[3], double r_in, double r_out, double out[3]) {
+  const double un = u[0] * xh[0] + u[1] * xh[1] + u[2] * xh[2];
+  const double ur = u[0] * rh[0] + u[1] * rh[1] + u[2] * rh[2];
+  double t[3] = {u[0] - un * xh[0], u[1] - un * xh[1], u[2] - un * xh[2]};
+  const double s = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+  double y[3] = {xh[0], xh[1], xh[2]};
+  if (s > 0) {
+    const double al = tau * s / r_ang, ca = cos(al), sa = sin(al);
+    for (int i = 0; i < 3; ++i) y[i] = ca * xh[i] - sa * (t[i] / s);
+  }
+  const double rn = fmin(fmax(r - tau * ur, r_in), r_out);
+  for (int i = 0; i < 3; ++i) out[i] = rn * y[i];
+}
+
+__global__ void k_sl_advect(AdvArgs a) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= a.nV) return;
+  const double x[3] = {a.X[3 * v], a.X[3 * v + 1], a.X[3 * v + 2]};
+  const double r = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  const double tol = 1e-9 * a.r_out;
+  if (fabs(r - a.r_in) < tol) { a.T_out[v] = a.T_cmb; return; }
+  if (fabs(r - a.r_out) < tol) { a.T_out[v] = a.T_surf; return; }
+  const double xh[3] = {x[0] / r, x[1] / r, x[2] / r};
+  const double u[3] = {a.uc[3 * v], a.uc[3 * v + 1], a.uc[3 * v + 2]};
+  double xm[3], l[4];
+  gc_back(xh, r, u, 0.5 * a.dt, r, xh, a.r_in, a.r_out, xm);
+  int32_t t = locate(a, v, xm, l);
+  double um[3] = {0, 0, 0};
+  for (int i = 0; i < 4; ++i) {
+    const int64_t w = a.tet_v[4 * (int64_t)t + i];
+    for (int k = 0; k < 3; ++k) um[k] += l[i] * a.uc[3 * w + k];
+  }
+  // radial velocity of the corrector in the predicted point's own radial
+  // direction: a rigid rotation then keeps the radius
+  const double rm = sqrt(xm[0] * xm[0] + xm[1] * xm[1] + xm[2] * xm[2]);
+  const double rmh[3] = {xm[0] / rm, xm[1] / rm, xm[2] / rm};
+  double xd[3];
+  gc_back(xh, r, um, a.dt, rm, rmh, a.r_in, a.r_out, xd);
+  t = locate(a, v, xd, l);
+  double Tv = 0;
+  for (int i = 0; i < 4; ++i) Tv += l[i] * a.T[a.tet_v[4 * (int64_t)t + i]];
+  a.T_out[v] = Tv;
+}
+
+inline int nb(int64_t n) { return (int)((n + 255) / 256); }
+
+void setup(m3_ctx *c) {
+  auto &A = c->adv;
+  const LevelDev &f = c->L[0];
+  const int64_t nV = f.nV, ne = f.nel;
+  std::vector<int32_t> tv(4 * ne);
+  std::vector<double> geo(12 * ne);
+  std::vector<int64_t> cnt(nV + 1, 0);
+  for (int64_t e = 0; e < ne; ++e) {
+    for (int i = 0; i < 4; ++i) {
+      tv[4 * e + i] = f.h_el2nod[10 * e + i];
+      ++cnt[tv[4 * e + i] + 1];
+    }
+    const double *p[4];
+    for (int i = 0; i < 4; ++i) p[i] = &c->h_coord[3 * (int64_t)tv[4 * e + i]];
+    double m[9];  // columns x_i - x_4
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 3; ++i) m[3 * i + j] = p[j][i] - p[3][i];
+    const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                       m[2] * (m[3] * m[7] - m[4] * m[6]);
+    M3_REQUIRE(det != 0, M3_E_MESH, "degenerate P1 tetrahedron");
+    double *g = &geo[12 * e];
+    g[0] = (m[4] * m[8] - m[5] * m[7]) / det;
+    g[1] = (m[2] * m[7] - m[1] * m[8]) / det;
+    g[2] = (m[1] * m[5] - m[2] * m[4]) / det;
+    g[3] = (m[5] * m[6] - m[3] * m[8]) / det;
+    g[4] = (m[0] * m[8] - m[2] * m[6]) / det;
+    g[5] = (m[2] * m[3] - m[0] * m[5]) / det;
+    g[6] = (m[3] * m[7] - m[4] * m[6]) / det;
+    g[7] = (m[1] * m[6] - m[0] * m[7]) / det;
+    g[8] = (m[0] * m[4] - m[1] * m[3]) / det;
+    for (int i = 0; i < 3; ++i) g[9 + i] = p[3][i];
+  }
+  for (int64_t v = 0; v < nV; ++v) {
+    M3_REQUIRE(cnt[v + 1] > 0, M3_E_MESH, "vertex without a tetrahedron");
+    cnt[v + 1] += cnt[v];
+  }
+  std::vector<int32_t> st(cnt[nV]);
+  std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int i = 0; i < 4; ++i) st[pos[tv[4 * e + i]]++] = (int32_t)e;
+  double rmin = 1e300, rmax = 0;
+  for (int64_t v = 0; v < nV; ++v) {
+    const double *x = &c->h_coord[3 * v];
+    const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    rmin = std::min(rmin, r);
+    rmax = std::max(rmax, r);
+  }
+  A.r_in = rmin;
+  A.r_out = rmax;
+  A.star_ptr.upload(cnt);
+  A.star_tet.upload(st);
+  A.tet_v.upload(tv);
+  A.tet_geo.upload(geo);
+  A.uc.alloc(3 * nV);
+  A.T.alloc(nV);
+  A.Tout.alloc(nV);
+  A.U.alloc(3 * nV);
+  A.ready = true;
+}
+
+}  // namespace
+
+namespace m3 {
+
+void sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,
+               double *T_out) {
+  M3_REQUIRE(!c->L.empty(), M3_E_STATE, "m3_sl_advect before m3_set_mesh");
+  M3_REQUIRE(T && U && T_out, M3_E_ARG, "null argument");
+  M3_REQUIRE(std::isfinite(dt) && dt >= 0, M3_E_ARG, "dt must be finite and non-negative");
+  const int64_t nV0 = c->L[0].nV;
+  for (int64_t i = 0; i < 3 * nV0; ++i)
+    M3_REQUIRE(std::isfinite(U[i]), M3_E_NUMERIC, "velocity contains NaN or Inf");
+  for (int64_t i = 0; i < nV0; ++i) M3_REQUIRE(std::isfinite(T[i]), M3_E_NUMERIC, "temperature contains NaN or Inf");
+  if (!c->adv.ready) setup(c);
+  auto &A = c->adv;
+  const int64_t nV = c->L[0].nV;
+  hipStream_t s = c->stream;
+  // vertex nodes are the first nV nodes: their rotated velocity is U[0 : 3 nV]
+  HIPCHECK(hipMemcpyAsync(A.U.p, U, sizeof(double) * 3 * nV, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(A.T.p, T, sizeof(double) * nV, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_vel_cart, dim3(nb(nV)), dim3(256), 0, s, nV, c->frames.p, A.U.p, A.uc.p);
+  AdvArgs a{nV, c->X.p, c->frames.p, A.U.p, A.T.p, A.star_ptr.p, A.star_tet.p, A.tet_v.p, A.tet_geo.p,
+            A.uc.p, A.r_in, A.r_out, dt, T_cmb, T_surf, A.Tout.p};
+  hipLaunchKernelGGL(k_sl_advect, dim3(nb(nV)), dim3(256), 0, s, a);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(T_out, A.Tout.p, sizeof(double) * nV, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace m3

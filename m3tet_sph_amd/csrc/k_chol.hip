@@ -348,6 +348,7 @@ void chol_analyse(m3_ctx *c) {
   LevelDev &cl = c->L[c->nmg - 1];
   CholTiles &ch = c->chol;
   const int64_t nn = cl.nnod;
+  HIPCHECK(hipStreamSynchronize(c->stream));  // plain copies do not wait for the solver stream
   const std::vector<double> mask = cl.mask.to_host();
   std::vector<int64_t> ptr(nn + 1, 0);
   std::vector<int32_t> adj;

@@ -1,0 +1,785 @@
+// Graph network of the learned warm start: featurisation, forward pass,
+// backward pass, Adam, warm-start assembly.  The network and the parameter
+// layout are described in include/m3gnn.h.
+//
+// Layout in HBM: vertex-major rows, h[i*64 + c] (BF16 bit patterns) for the
+// activations, FP32 for features, targets, gradients and parameters.  The
+// dense layers keep one output column per lane and that column of the weight
+// matrix in registers (128 VGPRs for a message-passing layer); the row
+// operand is broadcast from the lane that loaded it (v_readlane), so a wave
+// streams its rows from HBM once and never touches LDS.  Weight gradients
+// are block partial sums over row chunks (rows staged in LDS) followed by a
+// fixed-order reduction, which makes the gradient deterministic.  Every
+// product-accumulate is an FMA in a fixed order (bias first, then input
+// index ascending), which the CPU restatement follows.
+#include "m3_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/m3gnn.h"
+
+namespace {
+
+constexpr int NF = M3_GNN_NF, WD = M3_GNN_WIDTH, NO = M3_GNN_NOUT, NL = M3_GNN_LAYERS;
+constexpr int64_t OFF_WE = 0, OFF_BE = NF * WD, OFF_L = OFF_BE + WD, LSZ = 2 * WD * WD + WD,
+                  OFF_WD = OFF_L + NL * LSZ, OFF_BD = OFF_WD + WD * NO, NPARAM = OFF_BD + NO;
+static_assert(NPARAM == M3_GNN_NPARAM, "parameter layout");
+constexpr int NNORM = M3_GNN_NNORM;
+
+__device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ inline float ld(const float *p) { return *p; }
+__device__ inline float ld(const uint16_t *p) { return bf2f(*p); }
+__device__ inline float lanebc(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
+
+// ---------------------------------------------------------------- features
+__global__ void k_scales(int64_t n, const double *U, const double *P, double *sc) {
+  double su = 0, sp = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    su += U[3 * i] * U[3 * i] + U[3 * i + 1] * U[3 * i + 1] + U[3 * i + 2] * U[3 * i + 2];
+    sp += P[i] * P[i];
+  }
+  __shared__ double a[256], b[256];
+  a[threadIdx.x] = su;
+  b[threadIdx.x] = sp;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      a[threadIdx.x] += a[threadIdx.x + o];
+      b[threadIdx.x] += b[threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double u = sqrt(a[0] / (double)n), p = sqrt(b[0] / (double)n);
+    sc[0] = u > 0 ? u : 1.0;
+    sc[1] = p > 0 ? p : 1.0;
+  }
+}
+
+__global__ void k_features(int64_t n, const double *T, const double *Tp, const double *visc, const double *U,
+                           const double *P, const float *rn, const double *sc, float *X) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double su = sc[0], sp = sc[1];
+  float *x = X + NF * i;
+  x[0] = (float)T[i];
+  x[1] = (float)log(visc[i]);
+  x[2] = (float)(U[3 * i] / su);
+  x[3] = (float)(U[3 * i + 1] / su);
+  x[4] = (float)(U[3 * i + 2] / su);
+  x[5] = (float)(P[i] / sp);
+  x[6] = rn[i];
+  x[7] = (float)(T[i] - Tp[i]);
+}
+
+__global__ void k_targets(int64_t n, const double *U0, const double *P0, const double *U1, const double *P1,
+                          const double *sc, float *Y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double su = sc[0], sp = sc[1];
+  for (int c = 0; c < 3; ++c) Y[NO * i + c] = (float)((U1[3 * i + c] - U0[3 * i + c]) / su);
+  Y[NO * i + 3] = (float)((P1[i] - P0[i]) / sp);
+}
+
+// per-channel sums and sums of squares (C <= 8) over rows
+__global__ void k_chan_stats(int64_t rows, const float *X, int C, double *part) {
+  double s[8], q[8];
+  for (int c = 0; c < 8; ++c) s[c] = q[c] = 0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x)
+    for (int c = 0; c < C; ++c) {
+      const double v = X[r * C + c];
+      s[c] += v;
+      q[c] += v * v;
+    }
+  __shared__ double sh[16][256];
+  for (int c = 0; c < 8; ++c) {
+    sh[c][threadIdx.x] = s[c];
+    sh[8 + c][threadIdx.x] = q[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    double t = 0;
+    for (int k = 0; k < 256; ++k) t += sh[threadIdx.x][k];
+    part[16 * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
+// ---------------------------------------------------------------- forward
+// h0 = relu(xn We + be), xn = (x - mu) / sigma; one wave per vertex, lane = column
+__global__ __launch_bounds__(256) void k_enc(int64_t n, const float *X, const float *norm, const float *prm,
+                                             float *Xn, uint16_t *H0) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g >> 6;
+  const int c = (int)(g & 63);
+  if (i >= n) return;
+  float acc = prm[OFF_BE + c];
+  for (int k = 0; k < NF; ++k) {
+    const float xk = (X[NF * i + k] - norm[k]) / norm[NF + k];
+    acc = fmaf(xk, prm[OFF_WE + k * WD + c], acc);
+  }
+  if (c < NF) Xn[NF * i + c] = (X[NF * i + c] - norm[c]) / norm[NF + c];
+  H0[WD * i + c] = f2bf(fmaxf(acc, 0.f));
+}
+
+// m_i = mean of h over the neighbours of i (adjacency order)
+__global__ __launch_bounds__(256) void k_agg(int64_t n, const int64_t *ptr, const int32_t *col, const uint16_t *H,
+                                             uint16_t *M) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g >> 6;
+  const int c = (int)(g & 63);
+  if (i >= n) return;
+  const int64_t b = ptr[i], e = ptr[i + 1];
+  float s = 0.f;
+  for (int64_t p = b; p < e; ++p) s += bf2f(H[WD * (int64_t)col[p] + c]);
+  M[WD * i + c] = f2bf(e > b ? s / (float)(e - b) : 0.f);
+}
+
+// z = relu([h, m] W + b); h' = h + z   (W: 128 x 64 row-major, then b)
+__global__ __launch_bounds__(256) void k_mp(int64_t n, const uint16_t *H, const uint16_t *M, const float *Wl,
+                                            uint16_t *Z, uint16_t *Hn) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float wa[WD], wb[WD];
+#pragma unroll
+  for (int k = 0; k < WD; ++k) {
+    wa[k] = Wl[k * WD + lane];
+    wb[k] = Wl[(WD + k) * WD + lane];
+  }
+  const float bias = Wl[2 * WD * WD + lane];
+  for (int64_t i = w0; i < n; i += nw) {
+    const float h = bf2f(H[WD * i + lane]), m = bf2f(M[WD * i + lane]);
+    float acc = bias;
+#pragma unroll
+    for (int k = 0; k < WD; ++k) acc = fmaf(lanebc(h, k), wa[k], acc);
+#pragma unroll
+    for (int k = 0; k < WD; ++k) acc = fmaf(lanebc(m, k), wb[k], acc);
+    const float z = fmaxf(acc, 0.f);
+    Z[WD * i + lane] = f2bf(z);
+    Hn[WD * i + lane] = f2bf(h + z);
+  }
+}
+
+// y = h Wd + bd; lanes 0..3 each compute one output in input order
+__global__ __launch_bounds__(256) void k_dec(int64_t n, const uint16_t *H, const float *prm, float *Y) {
+  const int lane = threadIdx.x & 63;
+  const int o = lane & 3;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float wd[WD];
+#pragma unroll
+  for (int k = 0; k < WD; ++k) wd[k] = prm[OFF_WD + k * NO + o];
+  const float bias = prm[OFF_BD + o];
+  for (int64_t i = w0; i < n; i += nw) {
+    const float h = bf2f(H[WD * i + lane]);
+    float acc = bias;
+#pragma unroll
+    for (int k = 0; k < WD; ++k) acc = fmaf(lanebc(h, k), wd[k], acc);
+    if (lane < NO) Y[NO * i + lane] = acc;
+  }
+}
+
+// ---------------------------------------------------------------- backward
+// dY = 2 (y - t/out_sigma) / (n NO); per-block partial sums of squares
+__global__ void k_loss(int64_t n, const float *Y, const float *Tg, const float *norm, float scale, float *dY,
+                       double *part) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double s = 0;
+  if (e < n * NO) {
+    const int o = (int)(e % NO);
+    const float d = Y[e] - Tg[e] / norm[2 * NF + o];
+    dY[e] = d * scale;
+    s = (double)d * (double)d;
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// out[k] = sum(part[0:nb]) / denom, one block
+__global__ void k_loss_fin(int nbk, const double *part, double denom, float *out) {
+  __shared__ double sh[256];
+  double s = 0;
+  for (int b = threadIdx.x; b < nbk; b += blockDim.x) s += part[b];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = (float)(sh[0] / denom);
+}
+
+// dH[i][c] = sum_o dY[i][o] Wd[c][o]
+__global__ void k_dec_bwd(int64_t n, const float *dY, const float *prm, float *dH) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = g >> 6;
+  const int c = (int)(g & 63);
+  if (i >= n) return;
+  float acc = 0.f;
+  for (int o = 0; o < NO; ++o) acc = fmaf(dY[NO * i + o], prm[OFF_WD + c * NO + o], acc);
+  dH[WD * i + c] = acc;
+}
+
+// dP = dH where the stored activation is positive
+__global__ void k_mask(int64_t ne, const float *dH, const uint16_t *Z, float *dP) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  dP[e] = bf2f(Z[e]) > 0.f ? dH[e] : 0.f;
+}
+
+// dHn[i][k] = dH[i][k] + sum_c dP[i][c] W[k][c];  dM[i][k] = sum_c dP[i][c] W[64+k][c]
+__global__ __launch_bounds__(256) void k_mp_bwd(int64_t n, const float *dP, const float *Wl, const float *dH,
+                                                float *dHn, float *dM) {
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float wa[WD], wb[WD];
+#pragma unroll
+  for (int c = 0; c < WD; ++c) {
+    wa[c] = Wl[lane * WD + c];
+    wb[c] = Wl[(WD + lane) * WD + c];
+  }
+  for (int64_t i = w0; i < n; i += nw) {
+    const float p = dP[WD * i + lane];
+    float sa = 0.f, sb = 0.f;
+#pragma unroll
+    for (int c = 0; c < WD; ++c) {
+      const float pc = lanebc(p, c);
+      sa = fmaf(pc, wa[c], sa);
+      sb = fmaf(pc, wb[c], sb);
+    }
+    dHn[WD * i + lane] = dH[WD * i + lane] + sa;
+    dM[WD * i + lane] = sb;
+  }
+}
+
+// dH[j][c] += sum over neighbours i of j of dM[i][c] / deg(i)  (symmetric graph)
+__global__ __launch_bounds__(256) void k_agg_bwd(int64_t n, const int64_t *ptr, const int32_t *col, const float *dM,
+                                                 float *dH) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t j = g >> 6;
+  const int c = (int)(g & 63);
+  if (j >= n) return;
+  float s = 0.f;
+  for (int64_t p = ptr[j]; p < ptr[j + 1]; ++p) {
+    const int64_t i = col[p];
+    s += dM[WD * i + c] / (float)(ptr[i + 1] - ptr[i]);
+  }
+  dH[WD * j + c] += s;
+}
+
+// partial weight gradient of one row chunk: part[blk][k*N + c] = sum_r A[r][k] D[r][c]
+// for k < K1+K2 (A = [A1, A2]) and the bias row k = K1+K2 (A = 1)
+constexpr int TR = 32;
+constexpr int MAXJ = ((2 * WD + 1) * WD + 255) / 256;
+template <class TA>
+__global__ __launch_bounds__(256) void k_wgrad(int64_t n, int64_t chunk, const TA *A1, int K1, const uint16_t *A2,
+                                               int K2, const float *D, int N, float *part) {
+  __shared__ float As[TR * 2 * WD];
+  __shared__ float Ds[TR * WD];
+  const int K = K1 + K2, nout = (K + 1) * N;
+  const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+  float acc[MAXJ];
+#pragma unroll
+  for (int j = 0; j < MAXJ; ++j) acc[j] = 0.f;
+  for (int64_t rb = r0; rb < r1; rb += TR) {
+    const int nr = (int)min((int64_t)TR, r1 - rb);
+    __syncthreads();
+    for (int t = threadIdx.x; t < TR * K; t += blockDim.x) {
+      const int r = t / K, k = t % K;
+      float v = 0.f;
+      if (r < nr) v = k < K1 ? ld(A1 + (rb + r) * K1 + k) : bf2f(A2[(rb + r) * K2 + (k - K1)]);
+      As[t] = v;
+    }
+    for (int t = threadIdx.x; t < TR * N; t += blockDim.x) {
+      const int r = t / N, c = t % N;
+      Ds[t] = r < nr ? D[(rb + r) * N + c] : 0.f;
+    }
+    __syncthreads();
+    for (int r = 0; r < nr; ++r) {
+#pragma unroll
+      for (int j = 0; j < MAXJ; ++j) {
+        const int o = threadIdx.x + 256 * j;
+        if (o < nout) {
+          const int k = o / N, c = o % N;
+          const float a = k < K ? As[r * K + k] : 1.f;
+          acc[j] = fmaf(a, Ds[r * N + c], acc[j]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MAXJ; ++j) {
+    const int o = threadIdx.x + 256 * j;
+    if (o < nout) part[(int64_t)blockIdx.x * nout + o] = acc[j];
+  }
+}
+
+__global__ void k_wreduce(int nbk, int nout, const float *part, float *out) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  float s = 0.f;
+  for (int b = 0; b < nbk; ++b) s += part[(int64_t)b * nout + o];
+  out[o] = s;
+}
+
+__global__ void k_adam(int64_t np, float *p, const float *g, float *m, float *v, float lr, float c1, float c2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------- warm start
+__global__ void k_apply_vert(int64_t n, const float *Y, const float *norm, const double *sc, const double *Up,
+                             const double *Pp, double *dU, double *U0, double *P0) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double su = sc[0], sp = sc[1];
+  for (int c = 0; c < 3; ++c) {
+    const double d = (double)Y[NO * i + c] * (double)norm[2 * NF + c] * su;
+    dU[3 * i + c] = d;
+    U0[3 * i + c] = Up[3 * i + c] + d;
+  }
+  P0[i] = Pp[i] + (double)Y[NO * i + 3] * (double)norm[2 * NF + 3] * sp;
+}
+
+__global__ void k_apply_edge(int64_t nv, int64_t ne, const int32_t *par, const double *dU, const double *Up,
+                             double *U0) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= ne) return;
+  const int64_t a = par[2 * k], b = par[2 * k + 1], node = nv + k;
+  for (int c = 0; c < 3; ++c) U0[3 * node + c] = Up[3 * node + c] + 0.5 * (dU[3 * a + c] + dU[3 * b + c]);
+}
+
+}  // namespace
+
+// ==========================================================================
+struct m3_gnn {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t n = 0, nnod = 0;
+  DBuf<int64_t> ptr;
+  DBuf<int32_t> col, par;
+  DBuf<float> rn;
+  std::vector<float> h_prm, h_norm;
+  DBuf<float> prm, norm, grad, adam_m, adam_v;
+  int64_t adam_t = 0;
+  // activations and work arrays
+  DBuf<float> X, Xn, Y, Tg, dY, dH, dHn, dP, dM, part;
+  DBuf<uint16_t> H[NL + 1], M[NL + 1], Z[NL + 1];
+  DBuf<double> dpart, sc, dU;
+  DBuf<float> dloss;
+  // state inputs
+  DBuf<double> T, Tp, visc, Up, Pp, Un, Pn, U0, P0;
+  // training set
+  int32_t nsample = 0;
+  DBuf<float> sX, sY;
+};
+
+namespace {
+
+template <class F>
+int gguard(m3_gnn *g, F &&f) {
+  if (!g) return M3_E_ARG;
+  try {
+    HIPCHECK(hipSetDevice(g->device));
+    f();
+    g->err.clear();
+    return M3_OK;
+  } catch (const M3Error &e) {
+    g->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g->err = e.what();
+    return M3_E_ARG;
+  }
+}
+
+int wgrid(const m3_gnn *g) { return (int)std::min<int64_t>(256, (g->n + 63) / 64); }
+
+void forward(m3_gnn *g, const float *X) {
+  hipStream_t s = g->stream;
+  const int64_t n = g->n;
+  const int gw = (int)std::min<int64_t>(256, (n + 3) / 4);  // 4 waves per block, rows strided
+  hipLaunchKernelGGL(k_enc, dim3(nblk(64 * n)), dim3(256), 0, s, n, X, g->norm.p, g->prm.p, g->Xn.p, g->H[0].p);
+  for (int l = 1; l <= NL; ++l) {
+    hipLaunchKernelGGL(k_agg, dim3(nblk(64 * n)), dim3(256), 0, s, n, g->ptr.p, g->col.p, g->H[l - 1].p,
+                       g->M[l].p);
+    hipLaunchKernelGGL(k_mp, dim3(gw), dim3(256), 0, s, n, g->H[l - 1].p, g->M[l].p,
+                       g->prm.p + OFF_L + (l - 1) * LSZ, g->Z[l].p, g->H[l].p);
+  }
+  hipLaunchKernelGGL(k_dec, dim3(gw), dim3(256), 0, s, n, g->H[NL].p, g->prm.p, g->Y.p);
+}
+
+template <class TA>
+void wgrad(m3_gnn *g, const TA *A1, int K1, const uint16_t *A2, int K2, const float *D, int N, float *out) {
+  const int nbk = wgrid(g);
+  const int64_t chunk = (g->n + nbk - 1) / nbk;
+  const int nout = (K1 + K2 + 1) * N;
+  hipLaunchKernelGGL((k_wgrad<TA>), dim3(nbk), dim3(256), 0, g->stream, g->n, chunk, A1, K1, A2, K2, D, N,
+                     g->part.p);
+  hipLaunchKernelGGL(k_wreduce, dim3(nblk(nout)), dim3(256), 0, g->stream, nbk, nout, g->part.p, out);
+}
+
+// loss of the current prediction against Tg (raw targets); writes *dloss_slot
+void loss_and_backward(m3_gnn *g, const float *Tg, float *dloss_slot) {
+  hipStream_t s = g->stream;
+  const int64_t n = g->n, ne = NO * n;
+  const float scale = (float)(2.0 / (double)ne);
+  const int nb = nblk(ne);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, s, n, g->Y.p, Tg, g->norm.p, scale, g->dY.p, g->dpart.p);
+  hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(256), 0, s, nb, g->dpart.p, (double)ne, dloss_slot);
+  float *grad = g->grad.p;
+  wgrad<uint16_t>(g, g->H[NL].p, WD, nullptr, 0, g->dY.p, NO, grad + OFF_WD);
+  hipLaunchKernelGGL(k_dec_bwd, dim3(nblk(64 * n)), dim3(256), 0, s, n, g->dY.p, g->prm.p, g->dH.p);
+  const int gw = (int)std::min<int64_t>(256, (n + 3) / 4);
+  float *dH = g->dH.p, *dHn = g->dHn.p;
+  for (int l = NL; l >= 1; --l) {
+    const float *Wl = g->prm.p + OFF_L + (l - 1) * LSZ;
+    hipLaunchKernelGGL(k_mask, dim3(nblk(WD * n)), dim3(256), 0, s, WD * n, dH, g->Z[l].p, g->dP.p);
+    wgrad<uint16_t>(g, g->H[l - 1].p, WD, g->M[l].p, WD, g->dP.p, WD, grad + OFF_L + (l - 1) * LSZ);
+    hipLaunchKernelGGL(k_mp_bwd, dim3(gw), dim3(256), 0, s, n, g->dP.p, Wl, dH, dHn, g->dM.p);
+    hipLaunchKernelGGL(k_agg_bwd, dim3(nblk(64 * n)), dim3(256), 0, s, n, g->ptr.p, g->col.p, g->dM.p, dHn);
+    std::swap(dH, dHn);
+  }
+  hipLaunchKernelGGL(k_mask, dim3(nblk(WD * n)), dim3(256), 0, s, WD * n, dH, g->H[0].p, g->dP.p);
+  wgrad<float>(g, g->Xn.p, NF, nullptr, 0, g->dP.p, WD, grad + OFF_WE);
+}
+
+void adam_step(m3_gnn *g, float lr) {
+  ++g->adam_t;
+  const float c1 = 1.f - std::pow(0.9f, (float)g->adam_t), c2 = 1.f - std::pow(0.999f, (float)g->adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(NPARAM)), dim3(256), 0, g->stream, (int64_t)NPARAM, g->prm.p, g->grad.p,
+                     g->adam_m.p, g->adam_v.p, lr, c1, c2);
+}
+
+void upload_state(m3_gnn *g, const double *T, const double *Tp, const double *Visc, const double *Up,
+                  const double *Pp) {
+  M3_REQUIRE(T && Tp && Visc && Up && Pp, M3_E_ARG, "null argument");
+  hipStream_t s = g->stream;
+  const int64_t n = g->n;
+  for (int64_t i = 0; i < n; ++i)
+    M3_REQUIRE(Visc[i] > 0, M3_E_NUMERIC, "Calculated a zero or negative viscosity");
+  HIPCHECK(hipMemcpyAsync(g->T.p, T, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(g->Tp.p, Tp, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(g->visc.p, Visc, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(g->Up.p, Up, sizeof(double) * 3 * g->nnod, hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(g->Pp.p, Pp, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->sc.p);
+}
+
+void features_to(m3_gnn *g, float *X) {
+  hipLaunchKernelGGL(k_features, dim3(nblk(g->n)), dim3(256), 0, g->stream, g->n, g->T.p, g->Tp.p, g->visc.p,
+                     g->Up.p, g->Pp.p, g->rn.p, g->sc.p, X);
+}
+
+void require_params(const m3_gnn *g) {
+  M3_REQUIRE(!g->h_prm.empty(), M3_E_STATE, "network parameters not set (m3_gnn_set_params)");
+}
+
+}  // namespace
+
+extern "C" {
+
+int m3_gnn_create(m3_gnn **out, int hip_device, const m3_graph_desc *d) {
+  if (!out) return M3_E_ARG;
+  *out = nullptr;
+  m3_gnn *g = new m3_gnn();
+  g->device = hip_device;
+  const int rc = gguard(g, [&] {
+    int ndev = 0;
+    HIPCHECK(hipGetDeviceCount(&ndev));
+    M3_REQUIRE(ndev > 0, M3_E_HIP, "no HIP device visible");
+    M3_REQUIRE(hip_device >= 0 && hip_device < ndev, M3_E_ARG, "invalid HIP device index");
+    M3_REQUIRE(d && d->adj_ptr && d->adj_col && d->radius, M3_E_ARG, "null graph descriptor field");
+    const int64_t n = d->nvert, nn = d->nnod;
+    M3_REQUIRE(n > 0 && nn >= n, M3_E_ARG, "need nvert > 0 and nnod >= nvert");
+    M3_REQUIRE(nn == n || d->edge_parent, M3_E_ARG, "edge_parent required when nnod > nvert");
+    M3_REQUIRE(d->r_out > d->r_in, M3_E_ARG, "r_out must exceed r_in");
+    M3_REQUIRE(d->adj_ptr[0] == 0, M3_E_ARG, "adj_ptr[0] must be 0");
+    for (int64_t i = 0; i < n; ++i) {
+      M3_REQUIRE(d->adj_ptr[i + 1] > d->adj_ptr[i], M3_E_MESH, "vertex without neighbours");
+      for (int64_t p = d->adj_ptr[i]; p < d->adj_ptr[i + 1]; ++p) {
+        const int64_t j = d->adj_col[p];
+        M3_REQUIRE(j >= 0 && j < n && j != i, M3_E_MESH, "neighbour index out of range or self loop");
+        M3_REQUIRE(p == d->adj_ptr[i] || d->adj_col[p - 1] < j, M3_E_MESH, "neighbour lists must be sorted");
+      }
+    }
+    for (int64_t i = 0; i < n; ++i)  // symmetry: i in N(j) for every j in N(i)
+      for (int64_t p = d->adj_ptr[i]; p < d->adj_ptr[i + 1]; ++p) {
+        const int64_t j = d->adj_col[p];
+        const int32_t *b = d->adj_col + d->adj_ptr[j], *e = d->adj_col + d->adj_ptr[j + 1];
+        M3_REQUIRE(std::binary_search(b, e, (int32_t)i), M3_E_MESH, "graph must be symmetric");
+      }
+    for (int64_t k = 0; k < 2 * (nn - n); ++k)
+      M3_REQUIRE(d->edge_parent[k] >= 0 && d->edge_parent[k] < n, M3_E_MESH, "edge parent out of range");
+    HIPCHECK(hipSetDevice(hip_device));
+    HIPCHECK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    g->n = n;
+    g->nnod = nn;
+    g->ptr.upload(d->adj_ptr, n + 1);
+    g->col.upload(d->adj_col, d->adj_ptr[n]);
+    if (nn > n) g->par.upload(d->edge_parent, 2 * (nn - n));
+    std::vector<float> rn(n);
+    for (int64_t i = 0; i < n; ++i) rn[i] = (float)((d->radius[i] - d->r_in) / (d->r_out - d->r_in));
+    g->rn.upload(rn);
+    g->prm.alloc(NPARAM);
+    g->norm.alloc(NNORM);
+    g->grad.alloc(NPARAM);
+    g->adam_m.alloc(NPARAM);
+    g->adam_v.alloc(NPARAM);
+    g->X.alloc(NF * n);
+    g->Xn.alloc(NF * n);
+    g->Y.alloc(NO * n);
+    g->Tg.alloc(NO * n);
+    g->dY.alloc(NO * n);
+    for (auto *b : {&g->dH, &g->dHn, &g->dP, &g->dM}) b->alloc(WD * n);
+    for (int l = 0; l <= NL; ++l) {
+      g->H[l].alloc(WD * n);
+      if (l) {
+        g->M[l].alloc(WD * n);
+        g->Z[l].alloc(WD * n);
+      }
+    }
+    g->part.alloc((size_t)256 * (2 * WD + 1) * WD);
+    g->dpart.alloc(nblk(NO * n) + 16 * 256);
+    g->sc.alloc(2);
+    g->dU.alloc(3 * n);
+    g->dloss.alloc(1);
+    for (auto *b : {&g->T, &g->Tp, &g->visc, &g->Pp, &g->Pn, &g->P0}) b->alloc(n);
+    for (auto *b : {&g->Up, &g->Un, &g->U0}) b->alloc(3 * nn);
+  });
+  *out = g;
+  return rc;
+}
+
+void m3_gnn_destroy(m3_gnn *g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  hipStream_t s = g->stream;
+  g->stream = nullptr;
+  delete g;  // buffers are freed by their destructors
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char *m3_gnn_last_error(const m3_gnn *g) { return g ? g->err.c_str() : "null network"; }
+
+int m3_gnn_set_params(m3_gnn *g, const float *params, const float *norm) {
+  return gguard(g, [&] {
+    M3_REQUIRE(params, M3_E_ARG, "null parameters");
+    M3_REQUIRE(norm || !g->h_norm.empty(), M3_E_ARG, "null normalisation and none fitted");
+    if (norm) {
+      for (int k = 0; k < NF; ++k) M3_REQUIRE(norm[NF + k] > 0, M3_E_ARG, "in_sigma must be positive");
+      for (int k = 0; k < NO; ++k) M3_REQUIRE(norm[2 * NF + k] > 0, M3_E_ARG, "out_sigma must be positive");
+      g->h_norm.assign(norm, norm + NNORM);
+      g->norm.upload(g->h_norm);
+    }
+    g->h_prm.assign(params, params + NPARAM);
+    g->prm.upload(g->h_prm);
+    g->adam_m.zero(g->stream);
+    g->adam_v.zero(g->stream);
+    g->adam_t = 0;
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  });
+}
+
+int m3_gnn_get_params(m3_gnn *g, float *params, float *norm) {
+  return gguard(g, [&] {
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    if (params) {
+      require_params(g);
+      g->prm.download(params, NPARAM);
+    }
+    if (norm) {
+      M3_REQUIRE(!g->h_norm.empty(), M3_E_STATE, "normalisation not set");
+      g->norm.download(norm, NNORM);
+    }
+  });
+}
+
+int m3_gnn_features(m3_gnn *g, const double *T, const double *T_prev, const double *Visc, const double *U_prev,
+                    const double *P_prev, float *X, double *scales) {
+  return gguard(g, [&] {
+    upload_state(g, T, T_prev, Visc, U_prev, P_prev);
+    features_to(g, g->X.p);
+    HIPCHECK(hipGetLastError());
+    if (X) HIPCHECK(hipMemcpyAsync(X, g->X.p, sizeof(float) * NF * g->n, hipMemcpyDeviceToHost, g->stream));
+    if (scales) HIPCHECK(hipMemcpyAsync(scales, g->sc.p, sizeof(double) * 2, hipMemcpyDeviceToHost, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+  });
+}
+
+int m3_gnn_reserve(m3_gnn *g, int32_t nmax) {
+  return gguard(g, [&] {
+    M3_REQUIRE(nmax > 0, M3_E_ARG, "nsample_max must be positive");
+    g->sX.alloc((size_t)nmax * NF * g->n);
+    g->sY.alloc((size_t)nmax * NO * g->n);
+    g->nsample = nmax;
+  });
+}
+
+int m3_gnn_add_sample(m3_gnn *g, int32_t idx, const double *T, const double *T_prev, const double *Visc,
+                      const double *U_prev, const double *P_prev, const double *U_next, const double *P_next) {
+  return gguard(g, [&] {
+    M3_REQUIRE(idx >= 0 && idx < g->nsample, M3_E_ARG, "sample index out of range (m3_gnn_reserve)");
+    M3_REQUIRE(U_next && P_next, M3_E_ARG, "null argument");
+    upload_state(g, T, T_prev, Visc, U_prev, P_prev);
+    hipStream_t s = g->stream;
+    const int64_t n = g->n;
+    HIPCHECK(hipMemcpyAsync(g->Un.p, U_next, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(g->Pn.p, P_next, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    features_to(g, g->sX.p + (size_t)idx * NF * n);
+    hipLaunchKernelGGL(k_targets, dim3(nblk(n)), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->Un.p, g->Pn.p, g->sc.p,
+                       g->sY.p + (size_t)idx * NO * n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+int m3_gnn_fit_norm(m3_gnn *g, int32_t ns) {
+  return gguard(g, [&] {
+    M3_REQUIRE(ns > 0 && ns <= g->nsample, M3_E_ARG, "nsample out of range");
+    const int64_t rows = (int64_t)ns * g->n;
+    std::vector<float> nrm(NNORM);
+    for (int pass = 0; pass < 2; ++pass) {
+      const int C = pass ? NO : NF;
+      hipLaunchKernelGGL(k_chan_stats, dim3(256), dim3(256), 0, g->stream, rows, pass ? g->sY.p : g->sX.p, C,
+                         g->dpart.p);
+      HIPCHECK(hipGetLastError());
+      std::vector<double> part(16 * 256);
+      HIPCHECK(hipMemcpyAsync(part.data(), g->dpart.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                              g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      for (int c = 0; c < C; ++c) {
+        double s = 0, q = 0;
+        for (int b = 0; b < 256; ++b) {
+          s += part[16 * b + c];
+          q += part[16 * b + 8 + c];
+        }
+        const double mean = s / (double)rows, var = std::max(q / (double)rows - mean * mean, 0.0);
+        if (pass == 0) {
+          nrm[c] = (float)mean;
+          nrm[NF + c] = var > 1e-24 ? (float)std::sqrt(var) : 1.f;
+        } else {
+          const double rms = std::sqrt(q / (double)rows);
+          nrm[2 * NF + c] = rms > 1e-12 ? (float)rms : 1.f;
+        }
+      }
+    }
+    g->h_norm = nrm;
+    g->norm.upload(nrm);
+  });
+}
+
+int m3_gnn_train(m3_gnn *g, const int32_t *samples, int32_t nsteps, float lr, float *loss) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(samples && nsteps >= 0, M3_E_ARG, "bad sample list");
+    for (int32_t k = 0; k < nsteps; ++k)
+      M3_REQUIRE(samples[k] >= 0 && samples[k] < g->nsample, M3_E_ARG, "sample index out of range");
+    DBuf<float> dl;
+    dl.alloc(std::max(nsteps, 1));
+    const int64_t n = g->n;
+    for (int32_t k = 0; k < nsteps; ++k) {
+      const size_t sidx = (size_t)samples[k];
+      forward(g, g->sX.p + sidx * NF * n);
+      loss_and_backward(g, g->sY.p + sidx * NO * n, dl.p + k);
+      adam_step(g, lr);
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(g->stream));  // plain copies do not wait for this stream
+    if (loss && nsteps) dl.download(loss, nsteps);
+    g->h_prm.resize(NPARAM);
+    g->prm.download(g->h_prm.data(), NPARAM);
+  });
+}
+
+int m3_gnn_eval(m3_gnn *g, const int32_t *samples, int32_t ns, float *loss) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(samples && ns > 0 && loss, M3_E_ARG, "bad arguments");
+    const int64_t n = g->n, ne = NO * n;
+    double tot = 0;
+    for (int32_t k = 0; k < ns; ++k) {
+      M3_REQUIRE(samples[k] >= 0 && samples[k] < g->nsample, M3_E_ARG, "sample index out of range");
+      const size_t sidx = (size_t)samples[k];
+      forward(g, g->sX.p + sidx * NF * n);
+      const int nb = nblk(ne);
+      hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, g->stream, n, g->Y.p, g->sY.p + sidx * NO * n, g->norm.p,
+                         1.f, g->dY.p, g->dpart.p);
+      hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(256), 0, g->stream, nb, g->dpart.p, (double)ne, g->dloss.p);
+      float l = 0;
+      HIPCHECK(hipMemcpyAsync(&l, g->dloss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      tot += l;
+    }
+    *loss = (float)(tot / ns);
+  });
+}
+
+int m3_gnn_forward_backward(m3_gnn *g, const float *X, const float *Y, float *pred, float *loss, float *grad) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(X && Y, M3_E_ARG, "null argument");
+    const int64_t n = g->n;
+    g->X.upload(X, NF * n);
+    g->Tg.upload(Y, NO * n);
+    forward(g, g->X.p);
+    loss_and_backward(g, g->Tg.p, g->dloss.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    if (pred) g->Y.download(pred, NO * n);
+    if (loss) g->dloss.download(loss, 1);
+    if (grad) g->grad.download(grad, NPARAM);
+  });
+}
+
+int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const double *Visc, const double *U_prev,
+                      const double *P_prev, double *U0, double *P0) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(U0 && P0, M3_E_ARG, "null argument");
+    upload_state(g, T, T_prev, Visc, U_prev, P_prev);
+    hipStream_t s = g->stream;
+    const int64_t n = g->n, nn = g->nnod;
+    features_to(g, g->X.p);
+    forward(g, g->X.p);
+    hipLaunchKernelGGL(k_apply_vert, dim3(nblk(n)), dim3(256), 0, s, n, g->Y.p, g->norm.p, g->sc.p, g->Up.p,
+                       g->Pp.p, g->dU.p, g->U0.p, g->P0.p);
+    if (nn > n)
+      hipLaunchKernelGGL(k_apply_edge, dim3(nblk(nn - n)), dim3(256), 0, s, n, nn - n, g->par.p, g->dU.p, g->Up.p,
+                         g->U0.p);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(U0, g->U0.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(P0, g->P0.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+}  // extern "C"

@@ -187,6 +187,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
     const int nmg = m->nmg;
     c->nmg = nmg;
     c->L.clear();
+    c->adv.ready = false;
     c->L.resize(nmg);
     c->have_mesh = c->have_bc = c->assembled = c->restricted = c->factored = c->have_mixed = false;
     for (int l = 0; l < nmg; ++l) {
@@ -560,7 +561,7 @@ static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, 
   pres_res(true);
   double norm_rP = normdf(r, nV, (double)nV);
   const double norm_rP_init = norm_rP;
-  const double tol_Pat = rtol_Pat * norm_rP;
+  const double tol_Pat = o->tol_Pat_abs > 0 ? o->tol_Pat_abs : rtol_Pat * norm_rP;
   nitU += vel_solve(Uf, Rhs, rtol_U * rtol_restart * norm_rP, true);
   const double tol_U_abs = rtol_U * tol_Pat;
   int nitZ = 0, itPat = 0;
@@ -803,11 +804,17 @@ int m3_profile_read(m3_ctx *c, double *out, int32_t nout) {
   });
 }
 
+int m3_sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,
+                 double *T_out) {
+  return guard(c, [&] { m3::sl_advect(c, T, U, dt, T_cmb, T_surf, T_out); });
+}
+
 int m3_get_operator(m3_ctx *c, int32_t level, int64_t *rowptr, int32_t *col, double *val) {
   return guard(c, [&] {
     M3_REQUIRE(level >= 0 && level < c->nmg, M3_E_ARG, "bad level");
     M3_REQUIRE(level == 0 ? c->assembled : c->restricted, M3_E_STATE, "operator not built");
     Sell &K = c->L[level].K;
+    HIPCHECK(hipStreamSynchronize(c->stream));
     const std::vector<double> v = K.val.to_host();
     int64_t q = 0;
     rowptr[0] = 0;

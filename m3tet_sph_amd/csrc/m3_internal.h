@@ -196,6 +196,15 @@ struct m3_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[2];
   double prof_acc[2] = {0, 0};
   int64_t prof_n[2] = {0, 0};
+  // semi-Lagrangian temperature advection on the finest P1 tetrahedra
+  struct {
+    bool ready = false;
+    double r_in = 0, r_out = 0;
+    DBuf<int64_t> star_ptr;        // vertex -> incident tetrahedra (CSR)
+    DBuf<int32_t> star_tet, tet_v; // tet_v: 4 vertices per tetrahedron
+    DBuf<double> tet_geo;          // 12 per tetrahedron: inverse edge matrix, 4th vertex
+    DBuf<double> uc, T, Tout, U;   // Cartesian vertex velocity, temperatures, rotated U
+  } adv;
 };
 
 namespace m3 {
@@ -246,6 +255,8 @@ void chol_solve_dev(m3_ctx *c, const double *rhs, double *x);  // device vectors
 double dot(m3_ctx *c, const double *a, const double *b, int64_t n);
 double dot_diff(m3_ctx *c, const double *a, const double *b, const double *d, int64_t n);
 void axpy(hipStream_t s, double a, const double *x, double *y, int64_t n);
+void sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,
+               double *T_out);
 void xpby(hipStream_t s, const double *x, double b, double *y, int64_t n);  // y = x + b*y
 void copy(hipStream_t s, const double *x, double *y, int64_t n);
 void scale_mask(hipStream_t s, const double *x, const double *m, double *y, int64_t n);

@@ -1,0 +1,307 @@
+"""Learned warm start of the Stokes solve: host side.
+
+The reference starts each time step's Patera iteration from the previous
+step's velocity and pressure (flow3d_patera_mg_sph_p.m:204-231).  Here a
+graph network on the P1 vertex mesh predicts the change of (u, p) from the
+new temperature, the log-viscosity and the previous (u, p); the FP64-outer
+solver then iterates that starting point to the same residual tolerance.
+Everything that computes runs in ``libm3stokes.so`` (include/m3gnn.h): the
+featurisation, the forward and backward passes, Adam, and the warm-start
+assembly; this module builds the vertex graph, holds the ctypes binding and
+drives trajectory generation, training and evaluation.
+
+Training data come from a convection trajectory: plain FP64 solves along a
+sequence of temperature fields advanced by the semi-Lagrangian update
+(``StokesContext.sl_advect``, advect3d_slm_p_sph.m:1) at CFL 0.5.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+
+import numpy as np
+
+from . import api, fields
+from .api import LIB, M3Error, _p, D, I32, I64
+
+NF, WIDTH, NOUT, NLAYER = 8, 64, 4, 3
+NPARAM = NF * WIDTH + WIDTH + NLAYER * (2 * WIDTH * WIDTH + WIDTH) + WIDTH * NOUT + NOUT
+NNORM = 2 * NF + NOUT
+F32 = C.POINTER(C.c_float)
+
+# local edge of each P2 edge node (EL2NOD rows 5..10: edges 1-2, 2-3, 3-4, 4-1,
+# 1-3, 4-2; tetmesh_p1_to_p2_sph.m:574-580)
+EDGE_VERTS = ((0, 1), (1, 2), (2, 3), (3, 0), (0, 2), (3, 1))
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [("nvert", C.c_int64), ("nnod", C.c_int64), ("adj_ptr", I64), ("adj_col", I32),
+                ("edge_parent", I32), ("radius", D), ("r_in", C.c_double), ("r_out", C.c_double)]
+
+
+LIB.m3_gnn_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(GraphDesc)]
+LIB.m3_gnn_destroy.argtypes = [C.c_void_p]
+LIB.m3_gnn_destroy.restype = None
+LIB.m3_gnn_last_error.argtypes = [C.c_void_p]
+LIB.m3_gnn_last_error.restype = C.c_char_p
+LIB.m3_gnn_set_params.argtypes = [C.c_void_p, F32, F32]
+LIB.m3_gnn_get_params.argtypes = [C.c_void_p, F32, F32]
+LIB.m3_gnn_features.argtypes = [C.c_void_p, D, D, D, D, D, F32, D]
+LIB.m3_gnn_reserve.argtypes = [C.c_void_p, C.c_int32]
+LIB.m3_gnn_add_sample.argtypes = [C.c_void_p, C.c_int32, D, D, D, D, D, D, D]
+LIB.m3_gnn_fit_norm.argtypes = [C.c_void_p, C.c_int32]
+LIB.m3_gnn_train.argtypes = [C.c_void_p, I32, C.c_int32, C.c_float, F32]
+LIB.m3_gnn_eval.argtypes = [C.c_void_p, I32, C.c_int32, F32]
+LIB.m3_gnn_forward_backward.argtypes = [C.c_void_p, F32, F32, F32, F32, F32]
+LIB.m3_gnn_warm_start.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
+for _f in ("m3_gnn_create", "m3_gnn_set_params", "m3_gnn_get_params", "m3_gnn_features", "m3_gnn_reserve",
+           "m3_gnn_add_sample", "m3_gnn_fit_norm", "m3_gnn_train", "m3_gnn_eval", "m3_gnn_forward_backward",
+           "m3_gnn_warm_start"):
+    getattr(LIB, _f).restype = C.c_int
+
+
+def vertex_graph(mesh):
+    """CSR (sorted, symmetric, no self loops) of the edges of the finest P1
+    tetrahedra, the parents of every P2 edge node, and the vertex radii."""
+    E = mesh.levels[0].EL2NOD
+    nV, nn = mesh.nVnod, mesh.nnod
+    tets = E[:, :4].astype(np.int64)
+    pairs = np.concatenate([tets[:, [a, b]] for a in range(4) for b in range(4) if a != b])
+    key = np.unique(pairs[:, 0] * nV + pairs[:, 1])
+    rows, cols = key // nV, key % nV
+    ptr = np.zeros(nV + 1, dtype=np.int64)
+    np.add.at(ptr, rows + 1, 1)
+    ptr = np.cumsum(ptr)
+    parent = np.full((nn - nV, 2), -1, dtype=np.int32)
+    for k, (a, b) in enumerate(EDGE_VERTS):
+        node = E[:, 4 + k] - nV
+        parent[node, 0] = E[:, a]
+        parent[node, 1] = E[:, b]
+    if (parent < 0).any():
+        raise M3Error(3, "P2 node that is neither a vertex nor an edge node")
+    radius = np.linalg.norm(mesh.GCOORD[:nV], axis=1)
+    return ptr, cols.astype(np.int32), parent, radius
+
+
+def init_params(seed=0):
+    """He-scaled random weights; small residual branches and decoder, so that
+    the untrained network predicts nearly no change."""
+    rng = np.random.default_rng(seed)
+    p = []
+    p.append(rng.standard_normal((NF, WIDTH)) * np.sqrt(2.0 / NF))
+    p.append(np.zeros(WIDTH))
+    for _ in range(NLAYER):
+        p.append(rng.standard_normal((2 * WIDTH, WIDTH)) * np.sqrt(2.0 / (2 * WIDTH)) * 0.5)
+        p.append(np.zeros(WIDTH))
+    p.append(rng.standard_normal((WIDTH, NOUT)) * 0.01)
+    p.append(np.zeros(NOUT))
+    out = np.concatenate([a.ravel() for a in p]).astype(np.float32)
+    assert out.size == NPARAM
+    return out
+
+
+def identity_norm():
+    return np.concatenate([np.zeros(NF), np.ones(NF), np.ones(NOUT)]).astype(np.float32)
+
+
+class WarmStartNet:
+    """Device graph network over the vertex graph of ``mesh`` (include/m3gnn.h)."""
+
+    def __init__(self, mesh, device=0):
+        self.mesh = mesh
+        ptr, col, par, rad = vertex_graph(mesh)
+        self._keep = [np.ascontiguousarray(ptr), np.ascontiguousarray(col),
+                      np.ascontiguousarray(par.ravel()), np.ascontiguousarray(rad)]
+        d = GraphDesc()
+        d.nvert, d.nnod = mesh.nVnod, mesh.nnod
+        d.adj_ptr, d.adj_col = _p(self._keep[0], I64), _p(self._keep[1], I32)
+        d.edge_parent, d.radius = _p(self._keep[2], I32), _p(self._keep[3])
+        d.r_in, d.r_out = mesh.r_cmb, mesh.r_surf
+        self.n = mesh.nVnod
+        self._h = C.c_void_p()
+        rc = LIB.m3_gnn_create(C.byref(self._h), device, C.byref(d))
+        if rc:
+            msg = LIB.m3_gnn_last_error(self._h).decode() if self._h else "m3_gnn_create failed"
+            if self._h:
+                LIB.m3_gnn_destroy(self._h)
+            self._h = None
+            raise M3Error(rc, msg)
+        self.adj_ptr, self.adj_col, self.edge_parent, self.radius = ptr, col, par, rad
+
+    def close(self):
+        if self._h:
+            LIB.m3_gnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise M3Error(rc, LIB.m3_gnn_last_error(self._h).decode())
+
+    @staticmethod
+    def _d(a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def set_params(self, params, norm):
+        self.params = np.ascontiguousarray(params, dtype=np.float32)
+        self.norm = np.ascontiguousarray(norm, dtype=np.float32)
+        self._check(LIB.m3_gnn_set_params(self._h, _p(self.params, F32), _p(self.norm, F32)))
+
+    def get_params(self):
+        p = np.zeros(NPARAM, dtype=np.float32)
+        q = np.zeros(NNORM, dtype=np.float32)
+        self._check(LIB.m3_gnn_get_params(self._h, _p(p, F32), _p(q, F32)))
+        return p, q
+
+    def features(self, T, T_prev, Visc, U_prev, P_prev):
+        X = np.zeros((self.n, NF), dtype=np.float32)
+        sc = np.zeros(2)
+        a = [self._d(v) for v in (T, T_prev, Visc, U_prev, P_prev)]
+        self._check(LIB.m3_gnn_features(self._h, *[_p(v) for v in a], _p(X, F32), _p(sc)))
+        return X, sc
+
+    def reserve(self, nmax):
+        self._check(LIB.m3_gnn_reserve(self._h, int(nmax)))
+
+    def add_sample(self, idx, T, T_prev, Visc, U_prev, P_prev, U_next, P_next):
+        a = [self._d(v) for v in (T, T_prev, Visc, U_prev, P_prev, U_next, P_next)]
+        self._check(LIB.m3_gnn_add_sample(self._h, int(idx), *[_p(v) for v in a]))
+
+    def fit_norm(self, nsample):
+        self._check(LIB.m3_gnn_fit_norm(self._h, int(nsample)))
+
+    def train(self, samples, lr):
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        loss = np.zeros(max(len(s), 1), dtype=np.float32)
+        self._check(LIB.m3_gnn_train(self._h, _p(s, I32), len(s), float(lr), _p(loss, F32)))
+        return loss[:len(s)]
+
+    def eval(self, samples):
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(LIB.m3_gnn_eval(self._h, _p(s, I32), len(s), _p(loss, F32)))
+        return float(loss[0])
+
+    def forward_backward(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        pred = np.zeros((self.n, NOUT), dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        grad = np.zeros(NPARAM, dtype=np.float32)
+        self._check(LIB.m3_gnn_forward_backward(self._h, _p(X, F32), _p(Y, F32), _p(pred, F32), _p(loss, F32),
+                                                _p(grad, F32)))
+        return pred, float(loss[0]), grad
+
+    def warm_start(self, T, T_prev, Visc, U_prev, P_prev):
+        a = [self._d(v) for v in (T, T_prev, Visc, U_prev, P_prev)]
+        U0 = np.zeros(3 * self.mesh.nnod)
+        P0 = np.zeros(self.n)
+        self._check(LIB.m3_gnn_warm_start(self._h, *[_p(v) for v in a], _p(U0), _p(P0)))
+        return U0, P0
+
+    def save(self, path, meta=None):
+        p, q = self.get_params()
+        np.savez_compressed(path, params=p, norm=q, meta=json.dumps(meta or {}))
+
+    def load(self, path):
+        z = np.load(path, allow_pickle=False)
+        self.set_params(z["params"], z["norm"])
+        return json.loads(str(z["meta"])) if "meta" in z else {}
+
+
+# ---------------------------------------------------------------------------
+# trajectory, training, evaluation
+# ---------------------------------------------------------------------------
+
+def nodal_temperature(mesh, Tv, parent):
+    """P2 nodal temperature from vertex values (edge nodes: mean of the two
+    vertices; only vertex values enter the assembly, calc_element_visc.m:5-7)."""
+    T = np.empty(mesh.nnod)
+    T[:mesh.nVnod] = Tv
+    T[mesh.nVnod:] = 0.5 * (Tv[parent[:, 0]] + Tv[parent[:, 1]])
+    return T
+
+
+def min_edge_length(mesh, ptr, col):
+    X = mesh.GCOORD[:mesh.nVnod]
+    rows = np.repeat(np.arange(mesh.nVnod), np.diff(ptr))
+    return float(np.linalg.norm(X[rows] - X[col], axis=1).min())
+
+
+def time_step(U, nV, h_min, cfl=0.5):
+    """dt = cfl * h_min / max vertex speed (calculate_dt.m:40-56, advective part)."""
+    speed = np.linalg.norm(U[:3 * nV].reshape(-1, 3), axis=1).max()
+    return cfl * h_min / speed
+
+
+def generate_trajectory(mesh, ctx, net, nstates, visc_contrast=1e4, Ra=1e5, seed=2, cfl=0.5, tol=1e-7,
+                        keep_from=None, log=print):
+    """Plain FP64 solves along a semi-Lagrangian convection trajectory.
+    Sample k-1 of ``net``'s training set (k = 1..nstates-1) maps state k-1 and
+    the temperature of state k to the converged state k.  Returns per-state
+    temperatures, time steps, solver statistics and the full (U, P) of the
+    states with index >= keep_from."""
+    ptr, col, parent, _ = net.adj_ptr, net.adj_col, net.edge_parent, net.radius
+    h_min = min_edge_length(mesh, ptr, col)
+    phys = fields.physics(Ra)
+    nV = mesh.nVnod
+    Tv = fields.temperature(mesh, seed=seed)[:nV].copy()
+    keep_from = nstates if keep_from is None else keep_from
+    Ts, dts, stats, kept = [], [], [], {}
+    U_prev = P_prev = T_prev = None
+    rP_cold = None
+    net.reserve(max(nstates - 1, 1))
+    t0 = time.time()
+    for k in range(nstates):
+        Visc, Dens = fields.material(nodal_temperature(mesh, Tv, parent), visc_contrast)
+        if k == 0:
+            U, P, info = ctx.flow3d(Visc, Dens, phys, rtol_Pat=tol, rtol_U_final=tol)
+            rP_cold = info["norm_rP0"]
+        else:
+            U, P, info = ctx.flow3d(Visc, Dens, phys, P0=P_prev, U0=U_prev, rtol_U_final=tol,
+                                    tol_Pat_abs=tol * rP_cold)
+            net.add_sample(k - 1, Tv, T_prev, Visc, U_prev, P_prev, U, P)
+        Ts.append(Tv.copy())
+        stats.append(dict(nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"]))
+        if k >= keep_from - 1:
+            kept[k] = (U.copy(), P.copy())
+        dt = time_step(U, nV, h_min, cfl)
+        dts.append(dt)
+        if k % 25 == 0 or k == nstates - 1:
+            log(f"[trajectory] state {k}/{nstates}: nitP {info['nitP']} nitZ {info['nitZ']} dt {dt:.3e} "
+                f"Tmean {Tv.mean():.4f} ({time.time() - t0:.1f} s)", flush=True)
+        T_prev, U_prev, P_prev = Tv, U, P
+        Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
+    return dict(T=np.array(Ts), dt=np.array(dts), stats=stats, kept=kept, rP_cold0=rP_cold, h_min=h_min)
+
+
+def train(net, ntrain, nval_from, nval, epochs=100, lr=1e-3, lr_min=1e-5, seed=0, log=print):
+    """Adam over the training samples [0, ntrain) with a cosine learning-rate
+    schedule; validation loss on [nval_from, nval_from + nval)."""
+    rng = np.random.default_rng(seed)
+    net.fit_norm(ntrain)
+    net.set_params(init_params(seed), net_norm(net))
+    hist = []
+    val_ids = np.arange(nval_from, nval_from + nval)
+    t0 = time.time()
+    for ep in range(epochs):
+        lr_e = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * ep / max(epochs - 1, 1)))
+        loss = net.train(rng.permutation(ntrain), lr_e)
+        if ep % 10 == 0 or ep == epochs - 1:
+            vl = net.eval(val_ids) if nval else float("nan")
+            hist.append(dict(epoch=ep, lr=lr_e, train=float(loss.mean()), val=vl))
+            log(f"[train] epoch {ep}: train {loss.mean():.4e} val {vl:.4e} ({time.time() - t0:.1f} s)", flush=True)
+    return hist
+
+
+def net_norm(net):
+    """Normalisation fitted on the device (m3_gnn_fit_norm) as a host array."""
+    q = np.zeros(NNORM, dtype=np.float32)
+    net._check(LIB.m3_gnn_get_params(net._h, None, _p(q, F32)))
+    return q

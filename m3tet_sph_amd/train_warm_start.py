@@ -1,0 +1,138 @@
+"""Generate the convection trajectory, train the warm-start network on the
+GPU and evaluate it on held-out time steps.
+
+    python -m m3tet_sph_amd.train_warm_start --config c1 --states 501 \
+        --ntrain 400 --epochs 100 --out <dir>
+
+Trajectory: state 0 from the seeded temperature (seed 2), then for each
+state a plain FP64 solve (started, as in the reference's time loop, from the
+previous state) to a fixed residual, dt from CFL 0.5 on the vertex speeds,
+and the semi-Lagrangian temperature update.  Sample k-1 maps state k-1 and
+the temperature of state k to state k; samples [0, ntrain) train the
+network, the rest are held out.  For each held-out step the solve is run
+cold, from the previous state and from the network's prediction, in FP64
+and in the mixed-precision V-cycle, all to the same fixed residual
+(|r_U| <= 1e-6 |f_U| and |r_P| <= 1e-6 |r_P^cold|).
+
+Writes <out>/warm_start_<config>.npz (parameters, normalisation, metadata),
+<out>/<config>_heldout_states.npz (vertex temperatures of a few held-out
+steps and of their previous steps) and <out>/<config>_warm_start_eval.json.
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import api, fields, learned as L, mesh as M
+
+
+def log(*a, **k):
+    k.setdefault("flush", True)
+    print(*a, **k)
+
+
+def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol):
+    phys = fields.physics(Ra)
+    rows = []
+    t0 = time.time()
+    for k in steps:
+        T, Tp = traj["T"][k], traj["T"][k - 1]
+        Up, Pp = traj["kept"][k - 1]
+        V, Dn = fields.material(L.nodal_temperature(m, T, net.edge_parent), visc_contrast)
+        row = dict(step=int(k))
+
+        def run(name, prec, U0=None, P0=None, tol_abs=0.0, extra_s=0.0):
+            t = time.perf_counter()
+            if U0 is None:
+                U, P, info = ctx.flow3d(V, Dn, phys, precision=prec, rtol_Pat=tol, rtol_U_final=0.5 * tol)
+            else:
+                U, P, info = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, precision=prec, rtol_U_final=0.5 * tol,
+                                        tol_Pat_abs=tol_abs)
+            wall = time.perf_counter() - t + extra_s
+            rU, fU, rP = api.residual(ctx, U, P)
+            row[name] = dict(nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"], s=wall,
+                             rU_rel=rU / fU, norm_rP0=info["norm_rP0"], rP=rP)
+            return U, P, info
+
+        _, _, ic = run("cold_fp64", "fp64")
+        rP_cold = ic["norm_rP0"]
+        row["cold_fp64"]["rel_res"] = max(row["cold_fp64"]["rP"] / rP_cold, row["cold_fp64"]["rU_rel"])
+        tol_abs = tol * rP_cold
+        t = time.perf_counter()
+        U0, P0 = net.warm_start(T, Tp, V, Up, Pp)
+        t_net = time.perf_counter() - t
+        row["t_network_s"] = t_net
+        for name, prec, u0, p0, ex in (("prev_fp64", "fp64", Up, Pp, 0.0), ("net_fp64", "fp64", U0, P0, t_net),
+                                       ("prev_mixed", "mixed", Up, Pp, 0.0), ("net_mixed", "mixed", U0, P0, t_net)):
+            run(name, prec, u0, p0, tol_abs, ex)
+            row[name]["rel_res"] = max(row[name]["rP"] / rP_cold, row[name]["rU_rel"])
+        # error of the initial guesses against the converged FP64 solution
+        Uc, Pc, _ = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, rtol_U_final=1e-3 * tol, tol_Pat_abs=1e-3 * tol_abs)
+        row["init_err"] = dict(prev_U=float(np.linalg.norm(Up - Uc) / np.linalg.norm(Uc)),
+                               net_U=float(np.linalg.norm(U0 - Uc) / np.linalg.norm(Uc)),
+                               prev_P=float(np.linalg.norm(Pp - Pc) / np.linalg.norm(Pc)),
+                               net_P=float(np.linalg.norm(P0 - Pc) / np.linalg.norm(Pc)))
+        rows.append(row)
+        if len(rows) % 10 == 1 or k == steps[-1]:
+            log(f"[eval] step {k}: nitP cold {row['cold_fp64']['nitP']} prev {row['prev_fp64']['nitP']} "
+                f"net {row['net_fp64']['nitP']} | init err U prev {row['init_err']['prev_U']:.2e} "
+                f"net {row['init_err']['net_U']:.2e} ({time.time() - t0:.1f} s)")
+    keys = ("cold_fp64", "prev_fp64", "net_fp64", "prev_mixed", "net_mixed")
+    summary = {}
+    for kk in keys:
+        summary[kk] = {f: float(np.mean([r[kk][f] for r in rows])) for f in ("nitP", "nitZ", "nitU", "s", "rel_res")}
+        summary[kk]["max_rel_res"] = float(np.max([r[kk]["rel_res"] for r in rows]))
+    summary["init_err"] = {f: float(np.mean([r["init_err"][f] for r in rows]))
+                           for f in ("prev_U", "net_U", "prev_P", "net_P")}
+    return rows, summary
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--states", type=int, default=501)
+    ap.add_argument("--ntrain", type=int, default=400)
+    ap.add_argument("--epochs", type=int, default=100)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--visc-contrast", type=float, default=1e4)
+    ap.add_argument("--Ra", type=float, default=1e5)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--save-steps", type=int, default=3)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    t0 = time.time()
+    m = M.build_config(a.config)
+    log(f"[setup] mesh {a.config}: {m.nnod} P2 nodes, {m.nVnod} vertices ({time.time() - t0:.1f} s)")
+    ctx = api.StokesContext(m)
+    ifix, vfix = fields.plate_bcs(m, seed=1)
+    ctx.set_bc(ifix, vfix)
+    net = L.WarmStartNet(m)
+    traj = L.generate_trajectory(m, ctx, net, a.states, a.visc_contrast, a.Ra, a.seed, 0.5, 0.1 * a.tol,
+                                 keep_from=a.ntrain, log=log)
+    nsample = a.states - 1
+    hist = L.train(net, a.ntrain, a.ntrain, nsample - a.ntrain, a.epochs, a.lr, log=log)
+    meta = dict(config=a.config, states=a.states, ntrain=a.ntrain, epochs=a.epochs, lr=a.lr, seed=a.seed,
+                visc_contrast=a.visc_contrast, Ra=a.Ra, history=hist)
+    net.save(os.path.join(a.out, f"warm_start_{a.config}.npz"), meta)
+    held = list(range(a.ntrain + 1, a.states))
+    rows, summary = evaluate(m, ctx, net, traj, held, a.visc_contrast, a.Ra, a.tol)
+    save = np.linspace(held[0], held[-1], min(a.save_steps, len(held))).round().astype(int)
+    # FP32 copies keep the file small; they are the inputs of the held-out test and the benchmark
+    np.savez_compressed(os.path.join(a.out, f"{a.config}_heldout_states.npz"), steps=save,
+                        T=traj["T"][save].astype(np.float32), T_prev=traj["T"][save - 1].astype(np.float32),
+                        dt=traj["dt"][save - 1])
+    out = dict(meta=meta, summary=summary, per_step=rows,
+               trajectory=dict(dt=traj["dt"].tolist(), stats=traj["stats"], rP_cold0=traj["rP_cold0"],
+                               h_min=traj["h_min"]),
+               seconds=time.time() - t0)
+    with open(os.path.join(a.out, f"{a.config}_warm_start_eval.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    log("[summary] " + json.dumps(summary))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,158 @@
+"""GPU tests of the learned warm start and the semi-Lagrangian update:
+device kernels against the CPU restatements (oracle/gnn.py, oracle/advect.py),
+training progress, and the iteration gain of the trained network on held-out
+states of the c1 convection trajectory."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+from oracle import advect as OA
+from oracle import gnn as OG
+
+pytestmark = pytest.mark.gpu
+
+DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "m3tet_sph_amd", "data")
+
+
+@pytest.fixture(scope="module")
+def net0(c0):
+    from m3tet_sph_amd import learned as L
+    print("[gpu] WarmStartNet on c0", flush=True)
+    return L.WarmStartNet(c0)
+
+
+def _random_problem(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, OG.NF)).astype(np.float32)
+    Y = rng.standard_normal((n, OG.NO)).astype(np.float32)
+    p = (rng.standard_normal(OG.NPARAM) * 0.2).astype(np.float32)
+    norm = np.concatenate([rng.standard_normal(OG.NF) * 0.1, 1 + rng.random(OG.NF),
+                           0.5 + rng.random(OG.NO)]).astype(np.float32)
+    return X, Y, p, norm
+
+
+def test_forward_backward_matches_oracle(net0):
+    X, Y, p, norm = _random_problem(net0.n, 11)
+    net0.set_params(p, norm)
+    pred, loss, grad = net0.forward_backward(X, Y)
+    l_ref, g_ref, y_ref = OG.loss_and_grad(X, Y, p, norm, net0.adj_ptr, net0.adj_col, exact=True)
+    # same arithmetic in the same order: only a rare BF16 rounding tie can differ
+    assert relerr(pred, y_ref) < 1e-5
+    assert abs(loss - l_ref) <= 1e-5 * l_ref
+    blocks = [(OG.OFF_WE, OG.OFF_L)] + [(OG.OFF_L + l * OG.LSZ, OG.OFF_L + (l + 1) * OG.LSZ)
+                                         for l in range(OG.NL)] + [(OG.OFF_WD, OG.NPARAM)]
+    for a, b in blocks:
+        assert relerr(grad[a:b], g_ref[a:b]) < 1e-4, (a, b, relerr(grad[a:b], g_ref[a:b]))
+
+
+def test_features_and_warm_start_match_oracle(c0, c0_inputs, net0):
+    rng = np.random.default_rng(4)
+    nV, nn = c0.nVnod, c0.nnod
+    T = c0_inputs["T"][:nV]
+    Tp = T - 0.01 * rng.standard_normal(nV)
+    Up = rng.standard_normal(3 * nn)
+    Pp = rng.standard_normal(nV)
+    X, sc = net0.features(T, Tp, c0_inputs["Visc"], Up, Pp)
+    rn = (net0.radius - c0.r_cmb) / (c0.r_surf - c0.r_cmb)
+    Xr, scr = OG.features(T, Tp, c0_inputs["Visc"], Up, Pp, rn)
+    assert np.allclose(sc, scr, rtol=1e-13)
+    assert np.allclose(X, Xr, rtol=2e-7, atol=1e-7)
+    _, _, p, norm = _random_problem(nV, 12)
+    net0.set_params(p, norm)
+    U0, P0 = net0.warm_start(T, Tp, c0_inputs["Visc"], Up, Pp)
+    pred, _, _ = net0.forward_backward(X, np.zeros((nV, OG.NO), dtype=np.float32))
+    U0r, P0r = OG.warm_start(pred, norm, sc, Up, Pp, net0.edge_parent)
+    assert relerr(U0, U0r) < 1e-12 and relerr(P0, P0r) < 1e-12
+    # a network that predicts no change returns the previous state exactly
+    p0 = p.copy()
+    p0[OG.OFF_WD:] = 0
+    net0.set_params(p0, norm)
+    U0, P0 = net0.warm_start(T, Tp, c0_inputs["Visc"], Up, Pp)
+    assert np.array_equal(U0, Up) and np.array_equal(P0, Pp)
+
+
+def test_training_reduces_loss(c0, c0_inputs, net0):
+    """Targets that are a smooth function of the features are learned."""
+    from m3tet_sph_amd import learned as L
+    rng = np.random.default_rng(8)
+    nV, nn = c0.nVnod, c0.nnod
+    T0 = c0_inputs["T"]
+    net0.reserve(24)
+    for k in range(24):
+        T = T0[:nV] + 0.02 * rng.standard_normal() * np.sin(3 * c0.GCOORD[:nV, 0] + k)
+        Tp = T0[:nV]
+        Up = np.tile(rng.standard_normal(3), nn) * (1 + 0.1 * np.repeat(np.arange(nn) % 7, 3))
+        Pp = rng.standard_normal(nV)
+        dT = T - Tp
+        su = np.sqrt(np.mean(np.sum(Up[:3 * nV].reshape(-1, 3) ** 2, axis=1)))
+        sp = np.sqrt(np.mean(Pp ** 2))
+        Un = Up.copy()
+        Un[:3 * nV] += su * np.repeat(np.tanh(20 * dT), 3)
+        Pn = Pp + sp * 0.5 * np.sin(4 * T)
+        net0.add_sample(k, T, Tp, c0_inputs["Visc"], Up, Pp, Un, Pn)
+    net0.fit_norm(24)
+    net0.set_params(L.init_params(3), L.net_norm(net0))
+    first = net0.eval(np.arange(24))
+    for ep in range(60):
+        net0.train(rng.permutation(24), 2e-3)
+    last = net0.eval(np.arange(24))
+    print(f"[gpu] training loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.2 * first
+
+
+def test_sl_advect_matches_oracle(c0):
+    from m3tet_sph_amd import api
+    from m3tet_sph_amd.fields import rotation_frames
+    ctx = api.StokesContext(c0)
+    nV = c0.nVnod
+    X = c0.GCOORD[:nV]
+    tets = c0.levels[0].EL2NOD[:, :4].astype(np.int64)
+    omega = np.array([0.3, -0.2, 0.9])
+    uc = np.cross(omega[None, :], X) + 0.05 * X * np.sin(3 * X[:, :1])
+    R = rotation_frames(c0.GCOORD)
+    U = np.einsum("nji,nj->ni", R, np.vstack([uc, np.zeros((c0.nnod - nV, 3))])).ravel()
+    T = np.random.default_rng(2).random(nV)
+    elen = np.linalg.norm(X[tets[:, 0]] - X[tets[:, 1]], axis=1).min()
+    dt = 0.5 * elen / np.linalg.norm(uc, axis=1).max()
+    out = ctx.sl_advect(T, U, dt, 1.0, 0.0)
+    ref = OA.sl_advect(X, tets, T, uc, dt, 1.0, 0.0)
+    assert np.max(np.abs(out - ref)) < 1e-11
+    ctx.close()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(DATA, "warm_start_c1.npz")),
+                    reason="trained c1 network not present")
+def test_learned_warm_start_reduces_iterations_on_heldout_states():
+    """Held-out states of the c1 trajectory: the solve started from the
+    network's prediction reaches the same fixed residual in fewer pressure
+    iterations than the solve started from the previous step's state."""
+    from m3tet_sph_amd import api, fields, learned as L, mesh as M
+    m = M.build_config("c1")
+    z = np.load(os.path.join(DATA, "c1_heldout_states.npz"), allow_pickle=False)
+    ctx = api.StokesContext(m)
+    ifix, vfix = fields.plate_bcs(m, seed=1)
+    ctx.set_bc(ifix, vfix)
+    net = L.WarmStartNet(m)
+    net.load(os.path.join(DATA, "warm_start_c1.npz"))
+    phys = fields.physics(1e5)
+    tot = dict(prev=0, net=0)
+    for k in range(len(z["steps"])):
+        Tp, T = z["T_prev"][k], z["T"][k]
+        Vp, Dp = fields.material(L.nodal_temperature(m, Tp, net.edge_parent), 1e4)
+        Up, Pp, _ = ctx.flow3d(Vp, Dp, phys, rtol_Pat=1e-8, rtol_U_final=1e-8)
+        V, Dn = fields.material(L.nodal_temperature(m, T, net.edge_parent), 1e4)
+        _, _, ic = ctx.flow3d(V, Dn, phys, rtol_Pat=1e-6, rtol_U_final=1e-7)
+        tol = 1e-6 * ic["norm_rP0"]
+        U1, P1, i1 = ctx.flow3d(V, Dn, phys, P0=Pp, U0=Up, rtol_U_final=1e-7, tol_Pat_abs=tol)
+        U0, P0 = net.warm_start(T, Tp, V, Up, Pp)
+        U2, P2, i2 = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, rtol_U_final=1e-7, tol_Pat_abs=tol)
+        for U_, P_ in ((U1, P1), (U2, P2)):
+            rU, fU, rP = api.residual(ctx, U_, P_)
+            assert rU / fU < 1e-6 and rP / ic["norm_rP0"] < 1.5e-6
+        print(f"[gpu] held-out step {z['steps'][k]}: nitP prev {i1['nitP']} net {i2['nitP']}, "
+              f"nitZ prev {i1['nitZ']} net {i2['nitZ']}", flush=True)
+        tot["prev"] += i1["nitZ"] + i1["nitU"]
+        tot["net"] += i2["nitZ"] + i2["nitU"]
+    assert tot["net"] < tot["prev"], tot
