@@ -178,13 +178,13 @@ def main():
         Visc, Dens = inp["Visc"], inp["Dens"]
 
     def cold(form):
-        return ctx.flow3d(Visc, Dens, phys, P0=None, rtol_Pat=args.tol, rtol_U_final=0.5 * args.tol,
+        return ctx.flow3d(Visc, Dens, phys, P0=None, rtol_Pat=0.8 * args.tol, rtol_U_final=0.5 * args.tol,
                           precision=form)
 
     # fixed denominator: pressure residual of the velocity solve at P = 0
     U, P, info = cold("fp64")
     rP_cold = info["norm_rP0"]
-    tol_abs = args.tol * rP_cold
+    tol_abs = 0.8 * args.tol * rP_cold  # margin: the final velocity solve moves r_P slightly
 
     def warm(form, U0, P0):
         return ctx.flow3d(Visc, Dens, phys, P0=P0, U0=U0, precision=form, rtol_U_final=0.5 * args.tol,

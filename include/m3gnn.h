@@ -95,6 +95,12 @@ int m3_gnn_fit_norm(m3_gnn *g, int32_t nsample);
 int m3_gnn_train(m3_gnn *g, const int32_t *samples, int32_t nsteps, float lr, float *loss);
 /* Mean squared error of the listed samples under the current parameters. */
 int m3_gnn_eval(m3_gnn *g, const int32_t *samples, int32_t n, float *loss);
+/* Per output channel c, summed over the listed samples and vertices:
+ * out[3c] = sum y t, out[3c+1] = sum y y, out[3c+2] = sum t t, with y the
+ * normalised prediction and t the target divided by out_sigma.  The host
+ * uses them to fit a per-channel shrinkage of the decoder on calibration
+ * samples (y -> a_c y with a_c = sum y t / sum y y, clipped to [0, 1]). */
+int m3_gnn_sample_stats(m3_gnn *g, const int32_t *samples, int32_t n, double *out);
 
 /* One forward and backward pass on caller features X (n x 8) and raw targets
  * Y (n x 4, divided by out_sigma inside): prediction (n x 4, normalised),

@@ -238,6 +238,27 @@ __global__ void k_dec_bwd(int64_t n, const float *dY, const float *prm, float *d
   dH[WD * i + c] = acc;
 }
 
+// per-channel sums y*t, y*y, t*t (t = target / out_sigma); block partials
+__global__ void k_chan_dot(int64_t n, const float *Y, const float *Tg, const float *norm, double *part) {
+  double a[3 * NO];
+  for (int k = 0; k < 3 * NO; ++k) a[k] = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    for (int o = 0; o < NO; ++o) {
+      const double y = Y[NO * i + o], t = (double)(Tg[NO * i + o] / norm[2 * NF + o]);
+      a[3 * o] += y * t;
+      a[3 * o + 1] += y * y;
+      a[3 * o + 2] += t * t;
+    }
+  __shared__ double sh[3 * NO][256];
+  for (int k = 0; k < 3 * NO; ++k) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  if (threadIdx.x < 3 * NO) {
+    double t = 0;
+    for (int k = 0; k < 256; ++k) t += sh[threadIdx.x][k];
+    part[3 * NO * blockIdx.x + threadIdx.x] = t;
+  }
+}
+
 // dP = dH where the stored activation is positive
 __global__ void k_mask(int64_t ne, const float *dH, const uint16_t *Z, float *dP) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -717,6 +738,30 @@ int m3_gnn_train(m3_gnn *g, const int32_t *samples, int32_t nsteps, float lr, fl
     if (loss && nsteps) dl.download(loss, nsteps);
     g->h_prm.resize(NPARAM);
     g->prm.download(g->h_prm.data(), NPARAM);
+  });
+}
+
+int m3_gnn_sample_stats(m3_gnn *g, const int32_t *samples, int32_t ns, double *out) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(samples && ns > 0 && out, M3_E_ARG, "bad arguments");
+    const int64_t n = g->n;
+    constexpr int NB = 64;
+    std::vector<double> acc(3 * NO, 0.0), part(3 * NO * NB);
+    for (int32_t k = 0; k < ns; ++k) {
+      M3_REQUIRE(samples[k] >= 0 && samples[k] < g->nsample, M3_E_ARG, "sample index out of range");
+      const size_t sidx = (size_t)samples[k];
+      forward(g, g->sX.p + sidx * NF * n);
+      hipLaunchKernelGGL(k_chan_dot, dim3(NB), dim3(256), 0, g->stream, n, g->Y.p, g->sY.p + sidx * NO * n,
+                         g->norm.p, g->dpart.p);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(part.data(), g->dpart.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                              g->stream));
+      HIPCHECK(hipStreamSynchronize(g->stream));
+      for (int b = 0; b < NB; ++b)
+        for (int q = 0; q < 3 * NO; ++q) acc[q] += part[3 * NO * b + q];
+    }
+    for (int q = 0; q < 3 * NO; ++q) out[q] = acc[q];
   });
 }
 

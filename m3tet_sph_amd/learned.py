@@ -53,11 +53,12 @@ LIB.m3_gnn_add_sample.argtypes = [C.c_void_p, C.c_int32, D, D, D, D, D, D, D]
 LIB.m3_gnn_fit_norm.argtypes = [C.c_void_p, C.c_int32]
 LIB.m3_gnn_train.argtypes = [C.c_void_p, I32, C.c_int32, C.c_float, F32]
 LIB.m3_gnn_eval.argtypes = [C.c_void_p, I32, C.c_int32, F32]
+LIB.m3_gnn_sample_stats.argtypes = [C.c_void_p, I32, C.c_int32, D]
 LIB.m3_gnn_forward_backward.argtypes = [C.c_void_p, F32, F32, F32, F32, F32]
 LIB.m3_gnn_warm_start.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
 for _f in ("m3_gnn_create", "m3_gnn_set_params", "m3_gnn_get_params", "m3_gnn_features", "m3_gnn_reserve",
            "m3_gnn_add_sample", "m3_gnn_fit_norm", "m3_gnn_train", "m3_gnn_eval", "m3_gnn_forward_backward",
-           "m3_gnn_warm_start"):
+           "m3_gnn_warm_start", "m3_gnn_sample_stats"):
     getattr(LIB, _f).restype = C.c_int
 
 
@@ -188,6 +189,13 @@ class WarmStartNet:
         self._check(LIB.m3_gnn_eval(self._h, _p(s, I32), len(s), _p(loss, F32)))
         return float(loss[0])
 
+    def sample_stats(self, samples):
+        """Per channel (sum y t, sum y y, sum t t) over the listed samples."""
+        s = np.ascontiguousarray(samples, dtype=np.int32)
+        out = np.zeros(3 * NOUT)
+        self._check(LIB.m3_gnn_sample_stats(self._h, _p(s, I32), len(s), _p(out)))
+        return out.reshape(NOUT, 3)
+
     def forward_backward(self, X, Y):
         X = np.ascontiguousarray(X, dtype=np.float32)
         Y = np.ascontiguousarray(Y, dtype=np.float32)
@@ -281,10 +289,14 @@ def generate_trajectory(mesh, ctx, net, nstates, visc_contrast=1e4, Ra=1e5, seed
     return dict(T=np.array(Ts), dt=np.array(dts), stats=stats, kept=kept, rP_cold0=rP_cold, h_min=h_min)
 
 
-def train(net, ntrain, nval_from, nval, epochs=100, lr=1e-3, lr_min=1e-5, seed=0, log=print):
-    """Adam over the training samples [0, ntrain) with a cosine learning-rate
-    schedule; validation loss on [nval_from, nval_from + nval)."""
+def train(net, ntrain, nval_from, nval, epochs=100, lr=1e-3, lr_min=1e-5, seed=0, ncal=50, log=print):
+    """Adam over samples [0, ntrain - ncal) with a cosine learning-rate
+    schedule, then a per-channel shrinkage of the decoder fitted on the
+    calibration samples [ntrain - ncal, ntrain) (a channel the network does
+    not predict better than "no change" is scaled towards zero); validation
+    loss on [nval_from, nval_from + nval)."""
     rng = np.random.default_rng(seed)
+    nfit = ntrain - ncal
     net.fit_norm(ntrain)
     net.set_params(init_params(seed), net_norm(net))
     hist = []
@@ -292,11 +304,23 @@ def train(net, ntrain, nval_from, nval, epochs=100, lr=1e-3, lr_min=1e-5, seed=0
     t0 = time.time()
     for ep in range(epochs):
         lr_e = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * ep / max(epochs - 1, 1)))
-        loss = net.train(rng.permutation(ntrain), lr_e)
+        loss = net.train(rng.permutation(nfit), lr_e)
         if ep % 10 == 0 or ep == epochs - 1:
             vl = net.eval(val_ids) if nval else float("nan")
             hist.append(dict(epoch=ep, lr=lr_e, train=float(loss.mean()), val=vl))
             log(f"[train] epoch {ep}: train {loss.mean():.4e} val {vl:.4e} ({time.time() - t0:.1f} s)", flush=True)
+    alpha = np.ones(NOUT)
+    if ncal > 0:
+        st = net.sample_stats(np.arange(nfit, ntrain))
+        alpha = np.clip(st[:, 0] / np.maximum(st[:, 1], 1e-300), 0.0, 1.0)
+        p, _ = net.get_params()
+        o = NPARAM - WIDTH * NOUT - NOUT
+        p[o:o + WIDTH * NOUT] = (p[o:o + WIDTH * NOUT].reshape(WIDTH, NOUT) * alpha[None, :]).ravel()
+        p[NPARAM - NOUT:] *= alpha
+        net.set_params(p, net_norm(net))
+        vl = net.eval(val_ids) if nval else float("nan")
+        hist.append(dict(calibration=alpha.tolist(), val=vl))
+        log(f"[train] decoder shrinkage {np.round(alpha, 3).tolist()}, val {vl:.4e}", flush=True)
     return hist
 
 

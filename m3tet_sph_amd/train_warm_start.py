@@ -46,7 +46,7 @@ def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol):
         def run(name, prec, U0=None, P0=None, tol_abs=0.0, extra_s=0.0):
             t = time.perf_counter()
             if U0 is None:
-                U, P, info = ctx.flow3d(V, Dn, phys, precision=prec, rtol_Pat=tol, rtol_U_final=0.5 * tol)
+                U, P, info = ctx.flow3d(V, Dn, phys, precision=prec, rtol_Pat=0.8 * tol, rtol_U_final=0.5 * tol)
             else:
                 U, P, info = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, precision=prec, rtol_U_final=0.5 * tol,
                                         tol_Pat_abs=tol_abs)
@@ -59,7 +59,7 @@ def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol):
         _, _, ic = run("cold_fp64", "fp64")
         rP_cold = ic["norm_rP0"]
         row["cold_fp64"]["rel_res"] = max(row["cold_fp64"]["rP"] / rP_cold, row["cold_fp64"]["rU_rel"])
-        tol_abs = tol * rP_cold
+        tol_abs = 0.8 * tol * rP_cold  # margin: the final velocity solve moves r_P slightly
         t = time.perf_counter()
         U0, P0 = net.warm_start(T, Tp, V, Up, Pp)
         t_net = time.perf_counter() - t
