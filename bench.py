@@ -137,7 +137,10 @@ def main():
     ap.add_argument("--form", default="learned", choices=["learned", "fp64", "mixed"])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--visc-contrast", type=float, default=None,
+                    help="default per config: c0 1e3, c1 1e4, c2 1e5, c4 1e6")
     args = ap.parse_args()
+    vc = args.visc_contrast or {"c0": 1e3, "c1": 1e4, "c2": 1e5, "c4": 1e6}.get(args.config, 1e4)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU (the Stokes solve is single-device; "
               "multi-GPU sharding is a design note only). Use --gpus 1.", file=sys.stderr)
@@ -147,7 +150,9 @@ def main():
 
     t0 = time.perf_counter()
     mesh = M.build_config(args.config)
-    inp = F.make_inputs(mesh, visc_contrast=1e4, Ra=1e5, seed_T=0, seed_plates=1)
+    # c4 adds a Gaussian thermal plume (dT 0.3, radius 0.05) at the CMB (theta 0.9, phi 1.3)
+    plume = (0.3, 0.05, 0.9, 1.3) if args.config == "c4" else None
+    inp = F.make_inputs(mesh, visc_contrast=vc, Ra=1e5, seed_T=0, seed_plates=1, plume=plume)
     ctx = api.StokesContext(mesh)
     ctx.set_bc(inp["ifix"], inp["vfix"])
     phys = inp["phys"]
@@ -170,9 +175,9 @@ def main():
         step = int(z["steps"][-1])
         Tv = z["T"][-1].astype(np.float64)
         Tp = z["T_prev"][-1].astype(np.float64)
-        Vp, Dp = F.material(L.nodal_temperature(mesh, Tp, net.edge_parent), 1e4)
+        Vp, Dp = F.material(L.nodal_temperature(mesh, Tp, net.edge_parent), vc)
         U_prev, P_prev, _ = ctx.flow3d(Vp, Dp, phys, rtol_Pat=0.1 * args.tol, rtol_U_final=0.1 * args.tol)
-        Visc, Dens = F.material(L.nodal_temperature(mesh, Tv, net.edge_parent), 1e4)
+        Visc, Dens = F.material(L.nodal_temperature(mesh, Tv, net.edge_parent), vc)
         workload = f"held-out step {step} of the seeded convection trajectory (seed 2)"
     else:
         Visc, Dens = inp["Visc"], inp["Dens"]
@@ -262,7 +267,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: spherical-shell Stokes, {mesh.nel} P2 tets, "
                                f"{3 * mesh.nnod} velocity dofs, {mesh.nVnod} pressure dofs, "
-                               f"{mesh.nmg}-level MG, viscosity contrast 1e4; {workload}",
+                               f"{mesh.nmg}-level MG, viscosity contrast {vc:.0e}; {workload}",
                    "form": {"learned": "mixed+learned", "fp64": "plain fp64", "mixed": "mixed"}[args.form],
                    "rel_res_target": args.tol, "seeds": [0, 1, 2], "Ra": 1e5},
         "rel_res": float(f"{rr:.3e}"),

@@ -116,7 +116,9 @@ typedef struct {
   double rtol_U_final; /* 0: reference behaviour.  > 0: the final velocity solve
                           (flow3d_patera_mg_sph_p.m:630-632) also stops only once
                           |r_U| <= rtol_U_final |f_U| (fixed-denominator residual),
-                          with up to itmax_U_final iterations */
+                          with up to itmax_U_final iterations, and is tightened
+                          (up to 3 times, 10x each) until the pressure residual
+                          -GG' U of the final velocity also meets tol_Pat */
   int32_t itmax_U_final;
   double tol_Pat_abs;  /* 0: reference behaviour (tol_Pat = rtol_Pat * initial
                           pressure residual, flow3d_patera_mg_sph_p.m:436-437).

@@ -168,15 +168,21 @@ __global__ __launch_bounds__(256) void k_potrf(const int32_t *cols, int col0, co
     }
     __syncthreads();
   }
-  if (threadIdx.x < TS) {  // X = inv(L), column j by forward substitution
-    const int j = threadIdx.x;
+  {  // X = inv(L) row by row; 4 threads per column j split the sum over k
+    __shared__ double red[4][TS];
+    const int j = threadIdx.x & 63, q = threadIdx.x >> 6;
     for (int i = 0; i < TS; ++i) {
-      double s = (i == j) ? 1.0 : 0.0;
-      for (int k = j; k < i; ++k) s -= L[i][k] * Xs[k][j];
-      Xs[i][j] = (i >= j) ? s / L[i][i] : 0.0;
+      double s = 0;
+      for (int k = j + q; k < i; k += 4) s += L[i][k] * Xs[k][j];
+      red[q][j] = s;
+      __syncthreads();
+      if (q == 0) {
+        const double t = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
+        Xs[i][j] = (i >= j) ? ((i == j ? 1.0 : 0.0) - t) / L[i][i] : 0.0;
+      }
+      __syncthreads();
     }
   }
-  __syncthreads();
   double *Di = dinv + (int64_t)J * TT;
   double *DiT = dinvT + (int64_t)J * TT;
   for (int t = threadIdx.x; t < TT; t += 256) {
@@ -255,7 +261,7 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
 
 // out[row] += sign * sum_p M_p v[vcol_p]   (64-row gathers; one workgroup per
 // chunk of at most GCH items of a task, partial sums added atomically)
-constexpr int GCH = 4;
+constexpr int GCH = 2;
 __global__ __launch_bounds__(256) void k_gtask(int chunk0, const int32_t *crow, const int64_t *cbeg,
                                                const int64_t *cend, const int32_t *mat,
                                                const int32_t *vcol, const double *mats, const double *v,
@@ -268,8 +274,16 @@ __global__ __launch_bounds__(256) void k_gtask(int chunk0, const int32_t *crow, 
   for (int64_t p = cbeg[ck]; p < cend[ck]; ++p) {
     const double *M = mats + (int64_t)mat[p] * TT;
     const double *vv = v + (int64_t)vcol[p] * TS;
-#pragma unroll 4
-    for (int c = part; c < TS; c += 4) acc += M[i + TS * c] * vv[c];
+    // all 16 loads of this thread's columns in flight before the FMAs: the
+    // phases are latency-bound, not bandwidth-bound
+    double mv[TS / 4], xv[TS / 4];
+#pragma unroll
+    for (int j = 0; j < TS / 4; ++j) {
+      mv[j] = M[i + TS * (part + 4 * j)];
+      xv[j] = vv[part + 4 * j];
+    }
+#pragma unroll
+    for (int j = 0; j < TS / 4; ++j) acc += mv[j] * xv[j];
   }
   red[part][i] = acc;
   __syncthreads();

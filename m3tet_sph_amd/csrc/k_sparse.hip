@@ -53,13 +53,20 @@ __device__ inline auto ldv(const MT *p) {
   else return *p;
 }
 
-// y = K x   (optionally fused: r = r0 - K x; d_out = x + w Dinv r; dot partial)
-// mode bits: 1 write y, 2 compute r = r0 - y and write r_out, 4 write d_out,
-// 8 accumulate partial dot(x, y) into part[blockIdx]
-// TPR threads cooperate on one row (TPR=1 on the finest level; 4 on small
-// levels so that a 43k-row level still fills the 256 CUs); the k loop is
-// unrolled by two so that two independent gathers are in flight per thread.
-template <class MT, class VT, int MODE, int TPR>
+// streaming load (matrix values and columns are read once per sweep; keeping
+// them out of the way leaves the L2 to the gathered vector)
+template <bool NT, class T>
+__device__ inline T lds_(const T *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, class MT>
+__device__ inline auto ldm(const MT *p) {
+  if constexpr (std::is_same<MT, uint16_t>::value) return to_f<uint16_t>(lds_<NT>(p));
+  else return lds_<NT>(p);
+}
+
+template <class MT, class VT, int MODE, int TPR, int UNR, bool NT>
 __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const VT *__restrict__ x,
                                                 const VT *__restrict__ r0, const VT *__restrict__ dinv,
                                                 VT w, VT *__restrict__ y, VT *__restrict__ r_out,
@@ -75,25 +82,35 @@ __global__ __launch_bounds__(256) void k_smooth(SellT<MT> K, int64_t nrow, const
     const int64_t ns = K.nslot;
     const MT *v = K.val;
     int k = sub;
-    for (; k + TPR < len; k += 2 * TPR) {
-      const int64_t p = base + (int64_t)k * 64, q = p + 64 * TPR;
-      const int64_t cp = 3 * (int64_t)K.col[p], cq = 3 * (int64_t)K.col[q];
-      const A x0 = (A)x[cp], x1 = (A)x[cp + 1], x2 = (A)x[cp + 2];
-      const A z0 = (A)x[cq], z1 = (A)x[cq + 1], z2 = (A)x[cq + 2];
-      y0 += (A)ldv(v + p) * x0 + (A)ldv(v + ns + p) * x1 + (A)ldv(v + 2 * ns + p) * x2;
-      y1 += (A)ldv(v + 3 * ns + p) * x0 + (A)ldv(v + 4 * ns + p) * x1 + (A)ldv(v + 5 * ns + p) * x2;
-      y2 += (A)ldv(v + 6 * ns + p) * x0 + (A)ldv(v + 7 * ns + p) * x1 + (A)ldv(v + 8 * ns + p) * x2;
-      y0 += (A)ldv(v + q) * z0 + (A)ldv(v + ns + q) * z1 + (A)ldv(v + 2 * ns + q) * z2;
-      y1 += (A)ldv(v + 3 * ns + q) * z0 + (A)ldv(v + 4 * ns + q) * z1 + (A)ldv(v + 5 * ns + q) * z2;
-      y2 += (A)ldv(v + 6 * ns + q) * z0 + (A)ldv(v + 7 * ns + q) * z1 + (A)ldv(v + 8 * ns + q) * z2;
+    for (; k + (UNR - 1) * TPR < len; k += UNR * TPR) {
+      int64_t pp[UNR], cc[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        pp[u] = base + (int64_t)(k + u * TPR) * 64;
+        cc[u] = 3 * (int64_t)lds_<NT>(K.col + pp[u]);
+      }
+      A xx[UNR][3];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) xx[u][j] = (A)x[cc[u] + j];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        const int64_t p = pp[u];
+        y0 += (A)ldm<NT>(v + p) * xx[u][0] + (A)ldm<NT>(v + ns + p) * xx[u][1] + (A)ldm<NT>(v + 2 * ns + p) * xx[u][2];
+        y1 += (A)ldm<NT>(v + 3 * ns + p) * xx[u][0] + (A)ldm<NT>(v + 4 * ns + p) * xx[u][1] +
+              (A)ldm<NT>(v + 5 * ns + p) * xx[u][2];
+        y2 += (A)ldm<NT>(v + 6 * ns + p) * xx[u][0] + (A)ldm<NT>(v + 7 * ns + p) * xx[u][1] +
+              (A)ldm<NT>(v + 8 * ns + p) * xx[u][2];
+      }
     }
-    if (k < len) {
+    for (; k < len; k += TPR) {
       const int64_t p = base + (int64_t)k * 64;
-      const int64_t cc = 3 * (int64_t)K.col[p];
+      const int64_t cc = 3 * (int64_t)lds_<NT>(K.col + p);
       const A x0 = (A)x[cc], x1 = (A)x[cc + 1], x2 = (A)x[cc + 2];
-      y0 += (A)ldv(v + p) * x0 + (A)ldv(v + ns + p) * x1 + (A)ldv(v + 2 * ns + p) * x2;
-      y1 += (A)ldv(v + 3 * ns + p) * x0 + (A)ldv(v + 4 * ns + p) * x1 + (A)ldv(v + 5 * ns + p) * x2;
-      y2 += (A)ldv(v + 6 * ns + p) * x0 + (A)ldv(v + 7 * ns + p) * x1 + (A)ldv(v + 8 * ns + p) * x2;
+      y0 += (A)ldm<NT>(v + p) * x0 + (A)ldm<NT>(v + ns + p) * x1 + (A)ldm<NT>(v + 2 * ns + p) * x2;
+      y1 += (A)ldm<NT>(v + 3 * ns + p) * x0 + (A)ldm<NT>(v + 4 * ns + p) * x1 + (A)ldm<NT>(v + 5 * ns + p) * x2;
+      y2 += (A)ldm<NT>(v + 6 * ns + p) * x0 + (A)ldm<NT>(v + 7 * ns + p) * x1 + (A)ldm<NT>(v + 8 * ns + p) * x2;
     }
   }
   if (TPR > 1) {
@@ -434,39 +451,47 @@ inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) /
 namespace m3 {
 
 // threads per row for a level of n rows: one thread per row when the level
-// alone fills the GPU, four otherwise (M3_TPR_FINE / M3_TPR_COARSE override)
-static int tpr_for(int64_t n) {
-  static int fine = -1, coarse = -1;
-  if (fine < 0) {
-    const char *e1 = getenv("M3_TPR_FINE"), *e2 = getenv("M3_TPR_COARSE");
-    fine = e1 ? atoi(e1) : 1;
-    coarse = e2 ? atoi(e2) : 4;
-  }
-  return n >= 131072 ? fine : coarse;
-}
+// alone fills the GPU, four otherwise (on the 43,554-row c1 middle level,
+// 1 / 2 / 4 threads per row measured 2.78 / 2.86 / 2.95 solves/s)
+static int tpr_for(int64_t n) { return n >= 131072 ? 1 : 4; }
 
-template <class MT, class VT, int MODE>
-static void launch_smooth(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
-                          double w, VT *y, VT *r_out, VT *d_out, double *part) {
+template <class MT, class VT, int MODE, int UNR, bool NT>
+static void launch_smooth_u(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
+                            double w, VT *y, VT *r_out, VT *d_out, double *part) {
   const int t = tpr_for(n);
   if (MODE & 8) {  // partial sums are indexed per block of 256 rows
-    hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv, (VT)w,
-                       y, r_out, d_out, part);
+    hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1, UNR, NT>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
+                       (VT)w, y, r_out, d_out, part);
     return;
   }
   switch (t) {
     case 2:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 2>), dim3(nblk(2 * n)), dim3(256), 0, s, K, n, x, r0, dinv,
-                         (VT)w, y, r_out, d_out, part);
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 2, UNR, NT>), dim3(nblk(2 * n)), dim3(256), 0, s, K, n, x, r0,
+                         dinv, (VT)w, y, r_out, d_out, part);
       break;
     case 4:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 4>), dim3(nblk(4 * n)), dim3(256), 0, s, K, n, x, r0, dinv,
-                         (VT)w, y, r_out, d_out, part);
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 4, UNR, NT>), dim3(nblk(4 * n)), dim3(256), 0, s, K, n, x, r0,
+                         dinv, (VT)w, y, r_out, d_out, part);
       break;
     default:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
+      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1, UNR, NT>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
                          (VT)w, y, r_out, d_out, part);
   }
+}
+
+// Two blocks in flight per thread (unrolling by 3 or 4 measured no faster on
+// the c1 fine level).  Streaming loads of the matrix on the fine level (3-4%
+// faster there, 5% less HBM traffic); the middle levels' matrices stay
+// cache-resident between sweeps and keep cached loads (streaming loads made
+// them 10-28% slower).  Remapping blocks so that each XCD sweeps one
+// contiguous row range measured 2x slower and is not used.
+template <class MT, class VT, int MODE>
+static void launch_smooth(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
+                          double w, VT *y, VT *r_out, VT *d_out, double *part) {
+  if (n >= 131072)
+    launch_smooth_u<MT, VT, MODE, 2, true>(s, K, n, x, r0, dinv, w, y, r_out, d_out, part);
+  else
+    launch_smooth_u<MT, VT, MODE, 2, false>(s, K, n, x, r0, dinv, w, y, r_out, d_out, part);
 }
 
 void spmv_K(hipStream_t s, LevelDev &lv, const double *x, double *y) {
