@@ -43,6 +43,15 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel, measured with two rocprofv3
+# counter passes (FETCH_SIZE x 1.9957 + WRITE_SIZE x 1.0000, factors from a
+# plain copy of known size); recorded in profiles/c1_k_smooth_hbm_traffic.md.
+# Keyed by (config, kernel); other configs report null.
+MEASURED_TRAFFIC = {
+    ("c1", "k_smooth<float,float,6,1,2,true>"): 520891157,
+    ("c1", "k_smooth<double,double,6,1,2,true>"): 994093278,
+}
+
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
@@ -106,7 +115,7 @@ def cpu_baseline(ctx, mesh, inp, n_cg, seed=0):
                 t_solve_s=t_solve, t_assembly_s=t_asm, t_cg_iter_s=t_it, t_coarse_factor_s=t_fac)
 
 
-def roofline(ctx, mesh, api, run_once, form):
+def roofline(ctx, mesh, api, run_once, form, config):
     """Average duration of the fine-level fused sweep over one solve (HIP
     events on the solver's stream) and its algorithmic bytes."""
     api.profile(ctx, True)
@@ -121,9 +130,9 @@ def roofline(ctx, mesh, api, run_once, form):
     avg = tsw / max(nsw, 1)
     achieved = bpl / avg / 1e9
     log(f"sweep: {nsw} launches avg {avg * 1e6:.1f} us, {bpl / 1e6:.1f} MB -> {achieved:.0f} GB/s")
-    kern = "k_smooth<double,double,6,1>" if form == "fp64" else "k_smooth<float,float,6,1>"
+    kern = "k_smooth<double,double,6,1,2,true>" if form == "fp64" else "k_smooth<float,float,6,1,2,true>"
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=MEASURED_TRAFFIC.get((config, kern)),
                 kernel=f"{kern}: fine-level fused Jacobi sweep", launches=nsw,
                 avg_us=round(avg * 1e6, 2), bytes_per_launch=int(bpl))
 
@@ -243,7 +252,7 @@ def main():
     n_cg = int(np.mean([i["nitU"] + i["nitZ"] for i in infos]))
     log(f"{args.form}: {res}; rel_res={rr:.2e} (U {rru:.2e}, P {rrp:.2e}); last info {infos[-1]}")
 
-    roof = roofline(ctx, mesh, api, main_fn, vform)
+    roof = roofline(ctx, mesh, api, main_fn, vform, args.config)
 
     cpu = None
     if not args.no_cpu:

@@ -27,7 +27,7 @@ namespace {
 
 constexpr int TS = 64;
 constexpr int TT = TS * TS;
-constexpr int LEAF = 40;  // nodes per leaf block
+constexpr int LEAF = 80;  // nodes per leaf block (40 / 80 / 160: 309 / 306 / 305 ms per c1 solve)
 
 // ---- host: nested dissection into tile-aligned blocks ----------------------------------
 struct NDState {
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
 
 // out[row] += sign * sum_p M_p v[vcol_p]   (64-row gathers; one workgroup per
 // chunk of at most GCH items of a task, partial sums added atomically)
-constexpr int GCH = 2;
+constexpr int GCH = 1;  // 4 / 2 / 1 items per workgroup: 1 fastest (latency-bound)
 __global__ __launch_bounds__(256) void k_gtask(int chunk0, const int32_t *crow, const int64_t *cbeg,
                                                const int64_t *cend, const int32_t *mat,
                                                const int32_t *vcol, const double *mats, const double *v,
