@@ -116,9 +116,11 @@ typedef struct {
   double rtol_U_final; /* 0: reference behaviour.  > 0: the final velocity solve
                           (flow3d_patera_mg_sph_p.m:630-632) also stops only once
                           |r_U| <= rtol_U_final |f_U| (fixed-denominator residual),
-                          with up to itmax_U_final iterations, and is tightened
-                          (up to 3 times, 10x each) until the pressure residual
-                          -GG' U of the final velocity also meets tol_Pat */
+                          with up to itmax_U_final iterations; if the pressure
+                          residual -GG' U of that velocity misses tol_Pat, the
+                          velocity solve is tightened 10x once and, if still
+                          needed, the pressure iteration continues from the
+                          true residual (up to 3 times) */
   int32_t itmax_U_final;
   double tol_Pat_abs;  /* 0: reference behaviour (tol_Pat = rtol_Pat * initial
                           pressure residual, flow3d_patera_mg_sph_p.m:436-437).
@@ -143,8 +145,34 @@ int m3_set_mesh(m3_ctx *ctx, const m3_mesh_desc *mesh);
 int m3_set_bc(m3_ctx *ctx, const int64_t *ifix, const double *vfix, int64_t nfix);
 
 /* Assembly on the device: K1 = RR' K RR with the BC dofs eliminated, GG = RR' GG,
- * Fb = RR' Fb, MM, bc_rhs.  Visc/Dens are nodal (nnod). */
+ * Fb = RR' Fb, MM, bc_rhs.  Visc/Dens are nodal (nnod).  Visc = Dens = NULL
+ * (here and in m3_flow3d_patera_sph) uses the fields that m3_rock_prop left
+ * on the device. */
 int m3_assemble(m3_ctx *ctx, const double *Visc, const double *Dens, const m3_phys *phys);
+
+/* Material law on the device (mfiles_M3TET/rock_prop.m:31-118):
+ *   Dens = Dens0 (1 - alpha (T - T_ref)), clipped to [minDens, maxDens];
+ *   f_T  = exp(Ea/(R T0_K) (T0_K/T_K - 1)) clipped to [1e-8, 1e8] (Ea > 0),
+ *          T_K = T + 273, T0_K = T_ref + 273, R = 8.314472;
+ *   f_P  = f_T exp(|g Dens0 L0 depth| Va / (R T0_K)) (Va > 0),
+ *          depth = max node radius - radius;
+ *   f_LM = f_T a_UM2LM (a_UM2LM > 0);
+ *   Visc = Visc0 min(f_P, f_LM), clipped to [minVisc, maxVisc].
+ * T is nodal (nnod).  The fields stay on the device for the next assembly
+ * (pass Visc = Dens = NULL there); Visc/Dens outputs here may be NULL. */
+typedef struct {
+  double Visc0, Dens0, T_ref, alpha, Ea, Va, g, L0, a_UM2LM;
+  double minVisc, maxVisc, minDens, maxDens;
+  int32_t has_minVisc, has_maxVisc, has_minDens, has_maxDens;
+} m3_rock_params;
+int m3_rock_prop(m3_ctx *ctx, const double *T, const m3_rock_params *rp, double *Visc, double *Dens);
+
+/* Advective time step (mfiles_M3TET/calculate_dt.m:40-56):
+ *   dt = min(dtmax, dt_adv_limit * min(Lx/max|Ux|, Ly/max|Uy|, Lz/max|Uz|))
+ * with L the mesh extents (init_mesh_curved_p.m:270) and U (3*nnod, rotated
+ * frame as returned by m3_flow3d_patera_sph) turned to Cartesian on the
+ * device; dt = dtmax for a zero velocity. */
+int m3_calculate_dt(m3_ctx *ctx, const double *U, double dt_adv_limit, double dtmax, double *dt);
 
 /* Element-packed matrices in the reference layout and original Cartesian
  * frame: KKv 465 x nel, GGv 120 x nel, MMv 10 x nel, Fbv 30 x nel. */

@@ -100,6 +100,31 @@ LIB.m3_apply.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D, D]
 LIB.m3_get_vector.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D]
 LIB.m3_stats.argtypes = [C.c_void_p, I64, C.c_int32]
 LIB.m3_sl_advect.argtypes = [C.c_void_p, D, D, C.c_double, C.c_double, C.c_double, D]
+
+
+class RockParams(C.Structure):
+    """PHYSICS fields read by rock_prop.m:31-118 (m3_rock_params)."""
+    _fields_ = [(k, C.c_double) for k in ("Visc0", "Dens0", "T_ref", "alpha", "Ea", "Va", "g", "L0", "a_UM2LM",
+                                          "minVisc", "maxVisc", "minDens", "maxDens")] + \
+               [(k, C.c_int32) for k in ("has_minVisc", "has_maxVisc", "has_minDens", "has_maxDens")]
+
+
+def rock_params(Visc0=1.0, Dens0=1.0, T_ref=0.0, alpha=0.0, Ea=0.0, Va=0.0, g=-1.0, L0=1.0, a_UM2LM=0.0,
+                minVisc=None, maxVisc=None, minDens=None, maxDens=None):
+    p = RockParams()
+    for k, v in dict(Visc0=Visc0, Dens0=Dens0, T_ref=T_ref, alpha=alpha, Ea=Ea, Va=Va, g=g, L0=L0,
+                     a_UM2LM=a_UM2LM).items():
+        setattr(p, k, v)
+    for k, v in dict(minVisc=minVisc, maxVisc=maxVisc, minDens=minDens, maxDens=maxDens).items():
+        setattr(p, k, 0.0 if v is None else v)
+        setattr(p, "has_" + k, int(v is not None))
+    return p
+
+
+LIB.m3_rock_prop.argtypes = [C.c_void_p, D, C.POINTER(RockParams), D, D]
+LIB.m3_calculate_dt.argtypes = [C.c_void_p, D, C.c_double, C.c_double, D]
+LIB.m3_rock_prop.restype = C.c_int
+LIB.m3_calculate_dt.restype = C.c_int
 for _f in ("m3_create", "m3_set_mesh", "m3_set_bc", "m3_assemble", "m3_element_matrices",
            "m3_restrict_stiffmat", "m3_cholesky_factorization", "m3_cholesky_solve", "m3_mgpcg",
            "m3_patera", "m3_flow3d_patera_sph", "m3_apply", "m3_get_vector", "m3_stats",
@@ -109,6 +134,16 @@ for _f in ("m3_create", "m3_set_mesh", "m3_set_bc", "m3_assemble", "m3_element_m
 
 def _p(a, t=D):
     return a.ctypes.data_as(t)
+
+
+def _fields(Visc, Dens):
+    """Host pointers of the nodal fields, or NULL for the device-resident ones."""
+    if Visc is None and Dens is None:
+        return None, None
+    V = np.ascontiguousarray(Visc, dtype=np.float64)
+    R = np.ascontiguousarray(Dens, dtype=np.float64)
+    _fields.keep = (V, R)
+    return _p(V), _p(R)
 
 
 def phys_struct(phys, nip=14, type_Dmat="23rd"):
@@ -217,9 +252,9 @@ class StokesContext:
         self.ifix, self.vfix = np.asarray(ifix0), v
 
     def assemble(self, Visc, Dens, phys, type_Dmat="23rd"):
-        V = np.ascontiguousarray(Visc, dtype=np.float64)
-        R = np.ascontiguousarray(Dens, dtype=np.float64)
-        self._check(LIB.m3_assemble(self._h, _p(V), _p(R), C.byref(phys_struct(phys, type_Dmat=type_Dmat))))
+        """Visc = Dens = None: use the fields left on the device by rock_prop."""
+        V, R = _fields(Visc, Dens)
+        self._check(LIB.m3_assemble(self._h, V, R, C.byref(phys_struct(phys, type_Dmat=type_Dmat))))
 
     def element_matrices(self, Visc, Dens, phys, type_Dmat="23rd"):
         nel = self.mesh.nel
@@ -268,14 +303,32 @@ class StokesContext:
         n3 = 3 * self.mesh.nnod
         U = np.zeros(n3) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).copy()
         P = np.zeros(self.mesh.nVnod) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
-        V = np.ascontiguousarray(Visc, dtype=np.float64)
-        R = np.ascontiguousarray(Dens, dtype=np.float64)
+        V, R = _fields(Visc, Dens)
         info = Info()
         o = patera_opts(warm_U=U0 is not None, **opts)
-        self._check(LIB.m3_flow3d_patera_sph(self._h, _p(V), _p(R),
+        self._check(LIB.m3_flow3d_patera_sph(self._h, V, R,
                                              C.byref(phys_struct(phys, type_Dmat=type_Dmat)),
                                              _p(U), _p(P), C.byref(o), C.byref(info)))
         return U, P, info.as_dict()
+
+    def rock_prop(self, T, params, download=True):
+        """VAR = rock_prop(VAR,...) (rock_prop.m:31-118) on the device: nodal
+        viscosity and density from the nodal temperature.  The fields stay on
+        the device for the next ``assemble``/``flow3d`` called with
+        Visc = Dens = None; returned to the host if ``download``."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        V = np.zeros_like(T) if download else None
+        R = np.zeros_like(T) if download else None
+        self._check(LIB.m3_rock_prop(self._h, _p(T), C.byref(params), _p(V) if download else None,
+                                     _p(R) if download else None))
+        return V, R
+
+    def calculate_dt(self, U, dt_adv_limit=0.5, dtmax=np.inf):
+        """dt = calculate_dt(...) (calculate_dt.m:40-56) from the rotated-frame U."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        dt = C.c_double(0.0)
+        self._check(LIB.m3_calculate_dt(self._h, _p(U), float(dt_adv_limit), float(dtmax), C.byref(dt)))
+        return dt.value
 
     def sl_advect(self, T, U, dt, T_cmb=1.0, T_surf=0.0):
         """Semi-Lagrangian update of the vertex temperatures T (nVnod) over dt

@@ -190,6 +190,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
     c->adv.ready = false;
     c->L.resize(nmg);
     c->have_mesh = c->have_bc = c->assembled = c->restricted = c->factored = c->have_mixed = false;
+    c->have_fields = false;
     for (int l = 0; l < nmg; ++l) {
       LevelDev &lv = c->L[l];
       lv.nel = m->nel[l];
@@ -392,8 +393,13 @@ static void do_assemble(m3_ctx *c, const double *Visc, const double *Dens, const
   c->phys = *phys;
   LevelDev &f = c->L[0];
   hipStream_t s = c->stream;
-  HIPCHECK(hipMemcpyAsync(c->visc.p, Visc, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
-  HIPCHECK(hipMemcpyAsync(c->dens.p, Dens, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+  M3_REQUIRE((Visc == nullptr) == (Dens == nullptr), M3_E_ARG, "Visc and Dens must both be given or both be NULL");
+  if (Visc) {
+    HIPCHECK(hipMemcpyAsync(c->visc.p, Visc, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->dens.p, Dens, sizeof(double) * f.nnod, hipMemcpyHostToDevice, s));
+  } else {
+    M3_REQUIRE(c->have_fields, M3_E_STATE, "Visc = Dens = NULL needs m3_rock_prop first");
+  }
   f.K.val.zero(s);
   c->G.val.zero(s);
   c->GT.val.zero(s);
@@ -825,6 +831,14 @@ int m3_profile_read(m3_ctx *c, double *out, int32_t nout) {
     const double v[4] = {(double)c->prof_n[0], c->prof_acc[0], (double)c->prof_n[1], c->prof_acc[1]};
     for (int i = 0; i < nout && i < 4; ++i) out[i] = v[i];
   });
+}
+
+int m3_rock_prop(m3_ctx *c, const double *T, const m3_rock_params *rp, double *Visc, double *Dens) {
+  return guard(c, [&] { m3::rock_prop(c, T, rp, Visc, Dens); });
+}
+
+int m3_calculate_dt(m3_ctx *c, const double *U, double dt_adv_limit, double dtmax, double *dt) {
+  return guard(c, [&] { *dt = m3::calculate_dt(c, U, dt_adv_limit, dtmax); });
 }
 
 int m3_sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,

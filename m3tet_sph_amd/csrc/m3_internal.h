@@ -179,7 +179,7 @@ struct m3_ctx {
   std::vector<int64_t> h_ifix;
   std::vector<double> h_vfix;
   bool have_mesh = false, have_bc = false, assembled = false, restricted = false,
-       factored = false, have_mixed = false;
+       factored = false, have_mixed = false, have_fields = false;
   CholTiles chol;
   std::vector<double> h_coord;     // finest GCOORD row per node (for ordering)
   // reductions
@@ -257,6 +257,8 @@ double dot_diff(m3_ctx *c, const double *a, const double *b, const double *d, in
 void axpy(hipStream_t s, double a, const double *x, double *y, int64_t n);
 void sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,
                double *T_out);
+void rock_prop(m3_ctx *c, const double *T, const m3_rock_params *rp, double *Visc, double *Dens);
+double calculate_dt(m3_ctx *c, const double *U, double dt_adv_limit, double dtmax);
 void xpby(hipStream_t s, const double *x, double b, double *y, int64_t n);  // y = x + b*y
 void copy(hipStream_t s, const double *x, double *y, int64_t n);
 void scale_mask(hipStream_t s, const double *x, const double *m, double *y, int64_t n);
