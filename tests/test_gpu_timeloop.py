@@ -56,7 +56,8 @@ def test_device_fields_assemble_the_same_operator(c0, c0_inputs, ctx):
     ctx.rock_prop(T, rp, download=False)
     ctx.assemble(None, None, c0_inputs["phys"])
     K_dev = api.get_operator(ctx, 0).toarray()
-    assert np.array_equal(K_host, K_dev)
+    # same inputs; only the order of the FP64 atomic scatter differs
+    assert relerr(K_dev, K_host) < 1e-14
 
 
 def test_short_time_loop_on_device_fields(c0, c0_inputs, ctx):
