@@ -1,0 +1,121 @@
+"""Consecutive time steps of the convection loop with the Stokes hot path:
+solve (mixed-precision V-cycle, started from the previous step), CFL time
+step, semi-Lagrangian temperature update; optionally, at every step, a
+second solve of the same state started from the warm-start network, so the
+two starts are compared on identical inputs.
+
+    python -m m3tet_sph_amd.timeloop --config c4 --steps 50 --out <dir> \
+        [--net m3tet_sph_amd/data/warm_start_c1.npz]
+
+Per step it records iterations, seconds and the fixed-denominator residual
+max(|r_P|/|r_P^cold|, |r_U|/|f_U|) (r_P^cold from the cold solve of step 0),
+and the fine-level sweep bandwidth of one solve.  Writes
+<out>/timeloop_<config>.json.
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import api, fields, learned as L, mesh as M
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="c4")
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--form", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--visc-contrast", type=float, default=None)
+    ap.add_argument("--Ra", type=float, default=1e5)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--net", default=None, help="warm-start network (.npz) to compare against")
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    os.makedirs(a.out, exist_ok=True)
+    vc = a.visc_contrast or {"c0": 1e3, "c1": 1e4, "c2": 1e5, "c4": 1e6}.get(a.config, 1e4)
+    plume = (0.3, 0.05, 0.9, 1.3) if a.config == "c4" else None
+    t0 = time.time()
+    m = M.build_config(a.config)
+    ctx = api.StokesContext(m)
+    ifix, vfix = fields.plate_bcs(m, seed=1)
+    ctx.set_bc(ifix, vfix)
+    ptr, col, parent, _ = L.vertex_graph(m)
+    h_min = L.min_edge_length(m, ptr, col)
+    net = None
+    if a.net:
+        net = L.WarmStartNet(m)
+        net.load(a.net)
+    log(f"[setup] {a.config}: {3 * m.nnod} velocity dofs, contrast {vc:.0e} ({time.time() - t0:.1f} s)")
+    phys = fields.physics(a.Ra)
+    nV = m.nVnod
+    Tv = fields.temperature(m, seed=2, plume=plume)[:nV].copy()
+    U = P = Tp = None
+    rows = []
+    rP_cold = None
+    for k in range(a.steps + 1):
+        V, Dn = fields.material(L.nodal_temperature(m, Tv, parent), vc)
+        row = dict(step=k)
+        if k == 0:
+            t = time.perf_counter()
+            U, P, info = ctx.flow3d(V, Dn, phys, precision=a.form, rtol_Pat=0.8 * a.tol,
+                                    rtol_U_final=0.5 * a.tol)
+            row["cold"] = dict(s=time.perf_counter() - t, nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"])
+            rP_cold = info["norm_rP0"]
+        else:
+            tol_abs = 0.8 * a.tol * rP_cold
+            if net is not None:
+                t = time.perf_counter()
+                U0, P0 = net.warm_start(Tv, Tp, V, U, P)
+                Un, Pn, i2 = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, precision=a.form, rtol_U_final=0.5 * a.tol,
+                                        tol_Pat_abs=tol_abs)
+                rU, fU, rP = api.residual(ctx, Un, Pn)
+                row["net"] = dict(s=time.perf_counter() - t, nitP=i2["nitP"], nitZ=i2["nitZ"], nitU=i2["nitU"],
+                                  rel_res=max(rU / fU, rP / rP_cold))
+            t = time.perf_counter()
+            U, P, info = ctx.flow3d(V, Dn, phys, P0=P, U0=U, precision=a.form, rtol_U_final=0.5 * a.tol,
+                                    tol_Pat_abs=tol_abs)
+            row["prev"] = dict(s=time.perf_counter() - t, nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"])
+        rU, fU, rP = api.residual(ctx, U, P)
+        row["rel_res"] = max(rU / fU, rP / rP_cold)
+        dt = L.time_step(U, nV, h_min, 0.5)
+        row["dt"] = dt
+        rows.append(row)
+        log(f"[step {k}] " + json.dumps({kk: vv for kk, vv in row.items() if kk != "step"}))
+        Tp = Tv
+        Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
+    # sweep bandwidth of one more solve of the last state
+    api.profile(ctx, True)
+    ctx.flow3d(V, Dn, phys, P0=P, U0=U, precision=a.form, rtol_U_final=0.5 * a.tol,
+               tol_Pat_abs=0.8 * a.tol * rP_cold)
+    pr = api.profile_read(ctx)
+    api.profile(ctx, False)
+    nnzb = ctx.stats()["nnzb"][0]
+    vb = 4 if a.form == "mixed" else 8
+    sweep_bytes = nnzb * (9 * vb + 4) + m.nnod * 4 + m.nnod * 3 * 4 * vb
+    n_, t_ = (pr["n_mixed"], pr["t_mixed"]) if a.form == "mixed" else (pr["n_fp64"], pr["t_fp64"])
+    gbs = sweep_bytes / (t_ / max(n_, 1)) / 1e9
+    steps = rows[1:]
+    summ = dict(config=a.config, form=a.form, steps=a.steps, contrast=vc, velocity_dofs=3 * m.nnod,
+                prev=dict(s=float(np.mean([r["prev"]["s"] for r in steps])),
+                          nitP=float(np.mean([r["prev"]["nitP"] for r in steps])),
+                          nitZ=float(np.mean([r["prev"]["nitZ"] for r in steps]))),
+                max_rel_res=float(max(r["rel_res"] for r in rows)), sweep_GBs=gbs,
+                sweep_avg_us=t_ / max(n_, 1) * 1e6, cold_step0=rows[0]["cold"])
+    if net is not None:
+        summ["net"] = dict(s=float(np.mean([r["net"]["s"] for r in steps])),
+                           nitP=float(np.mean([r["net"]["nitP"] for r in steps])),
+                           nitZ=float(np.mean([r["net"]["nitZ"] for r in steps])),
+                           max_rel_res=float(max(r["net"]["rel_res"] for r in steps)))
+    with open(os.path.join(a.out, f"timeloop_{a.config}.json"), "w") as f:
+        json.dump(dict(summary=summ, per_step=rows, seconds=time.time() - t0), f, indent=1)
+    log("[summary] " + json.dumps(summ))
+
+
+if __name__ == "__main__":
+    main()
