@@ -136,61 +136,70 @@ __global__ __launch_bounds__(256) void k_update(const int32_t *chunk_tile, const
     for (int j = 0; j < 4; ++j) atomicAdd(C + (4 * ty + i) + TS * (4 * tx + j), -acc[i][j]);
 }
 
-// Cholesky of the diagonal tile (in LDS) and inverse of the factor.
-__global__ __launch_bounds__(256) void k_potrf(const int32_t *cols, int col0, const int32_t *diagtile,
-                                               const int32_t *pos2dof, double *tiles, double *dinv,
-                                               double *dinvT, int *err) {
+// 64-bit value of lane `lane` (a compile-time constant in the unrolled loops)
+__device__ inline double bcast_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// Cholesky of the diagonal tile and inverse of the factor, one wavefront per
+// tile.  Lane t keeps row t of the tile in registers; step k broadcasts
+// L[k][k] and the column entries L[c][k] from their owning lanes
+// (v_readlane), so the elimination needs no LDS traffic and no barriers.
+// Only the lower triangle is meaningful; the upper part of each lane's row is
+// never read.  The inverse is then formed column by column (lane j, column j)
+// by forward substitution against L staged once in LDS.
+__global__ __launch_bounds__(64) void k_potrf(const int32_t *cols, int col0, const int32_t *diagtile,
+                                              const int32_t *pos2dof, double *tiles, double *dinv,
+                                              double *dinvT, int *err) {
   const int J = cols[col0 + blockIdx.x];
   double *A = tiles + (int64_t)diagtile[J] * TT;
-  __shared__ double L[TS][TS + 1];
-  __shared__ double Xs[TS][TS + 1];
-  for (int t = threadIdx.x; t < TT; t += 256) {
-    const int r = t & 63, c = t >> 6;
-    double v = A[t];
-    const bool pr = pos2dof[J * TS + r] < 0, pc = pos2dof[J * TS + c] < 0;
-    if (pr || pc) v = (r == c) ? 1.0 : 0.0;
-    L[r][c] = v;
+  __shared__ double S[TS][TS + 1];
+  const int t = threadIdx.x;
+  const bool pr = pos2dof[J * TS + t] < 0;
+  double row[TS];
+#pragma unroll
+  for (int c = 0; c < TS; ++c) {
+    double v = A[t + TS * c];
+    if (pr || pos2dof[J * TS + c] < 0) v = (t == c) ? 1.0 : 0.0;
+    row[c] = v;
+  }
+#pragma unroll
+  for (int k = 0; k < TS; ++k) {
+    const double d = bcast_lane(row[k], k);
+    if (t == 0 && !(d > 0)) atomicOr(err, 16);
+    const double dk = sqrt(d > 0 ? d : 1.0);
+    row[k] = (t == k) ? dk : row[k] / dk;
+#pragma unroll
+    for (int c = k + 1; c < TS; ++c) row[c] -= row[k] * bcast_lane(row[k], c);
+  }
+#pragma unroll
+  for (int c = 0; c < TS; ++c) {
+    const double v = (t >= c) ? row[c] : 0.0;
+    A[t + TS * c] = v;
+    S[t][c] = v;
   }
   __syncthreads();
-  for (int k = 0; k < TS; ++k) {
-    if (threadIdx.x == 0) {
-      const double d = L[k][k];
-      if (!(d > 0)) atomicOr(err, 16);
-      L[k][k] = sqrt(d > 0 ? d : 1.0);
-    }
-    __syncthreads();
-    const double dk = L[k][k];
-    for (int r = k + 1 + threadIdx.x; r < TS; r += 256) L[r][k] /= dk;
-    __syncthreads();
-    for (int t = threadIdx.x; t < TT; t += 256) {
-      const int r = t & 63, c = t >> 6;
-      if (c > k && r >= c) L[r][c] -= L[r][k] * L[c][k];
-    }
-    __syncthreads();
-  }
-  {  // X = inv(L) row by row; 4 threads per column j split the sum over k
-    __shared__ double red[4][TS];
-    const int j = threadIdx.x & 63, q = threadIdx.x >> 6;
-    for (int i = 0; i < TS; ++i) {
-      double s = 0;
-      for (int k = j + q; k < i; k += 4) s += L[i][k] * Xs[k][j];
-      red[q][j] = s;
-      __syncthreads();
-      if (q == 0) {
-        const double t = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
-        Xs[i][j] = (i >= j) ? ((i == j ? 1.0 : 0.0) - t) / L[i][i] : 0.0;
-      }
-      __syncthreads();
-    }
+  double x[TS];  // column t of inv(L)
+#pragma unroll
+  for (int i = 0; i < TS; ++i) {
+    double s = (i == t) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= S[i][k] * x[k];
+    x[i] = (i >= t) ? s / S[i][i] : 0.0;
   }
   double *Di = dinv + (int64_t)J * TT;
   double *DiT = dinvT + (int64_t)J * TT;
-  for (int t = threadIdx.x; t < TT; t += 256) {
-    const int r = t & 63, c = t >> 6;
-    A[t] = (r >= c) ? L[r][c] : 0.0;
-    Di[t] = Xs[r][c];
-    DiT[t] = Xs[c][r];
-  }
+#pragma unroll
+  for (int c = 0; c < TS; ++c) DiT[t + TS * c] = x[c];  // DiT[r + 64c] = X[c][r]
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < TS; ++i) S[i][t] = x[i];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < TS; ++c) Di[t + TS * c] = S[t][c];
 }
 
 // L_IJ = C_IJ inv(L_JJ)'  and its transposed copy
@@ -663,7 +672,7 @@ void chol_factor(m3_ctx *c) {
       hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(256), 0, s, ch.upd_tile.p, ch.upd_pair_ptr.p,
                          ch.upd_pair_end.p, ch.upd_pairs.p, u0, ch.tiles.p);
     const int c0 = ch.h_lev_ptr[l], c1 = ch.h_lev_ptr[l + 1];
-    hipLaunchKernelGGL(k_potrf, dim3(c1 - c0), dim3(256), 0, s, ch.lev_cols.p, c0, ch.diagtile.p,
+    hipLaunchKernelGGL(k_potrf, dim3(c1 - c0), dim3(64), 0, s, ch.lev_cols.p, c0, ch.diagtile.p,
                        ch.pos2dof.p, ch.tiles.p, ch.dinv.p, ch.dinvT.p, c->errflag.p);
     const int t0 = ch.h_trsm_ptr[l], t1 = ch.h_trsm_ptr[l + 1];
     if (t1 > t0)
