@@ -96,25 +96,58 @@ __global__ void k_gather(int64_t n, const int64_t *src, const int64_t *dst, cons
   if (i < n) tiles[dst[i]] = val[src[i]];
 }
 
-// acc(64x64, 4x4 per thread) += A * B' or A * B over a list of tile pairs
-__device__ inline void tile_mm(const double *A, const double *B, bool transB, double (*sA)[TS + 1],
-                               double (*sB)[TS + 1], double acc[4][4]) {
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+// acc(64x64) += A * B' (transB) or A * B over a list of tile pairs, A and B
+// 64x64 column-major tiles staged in LDS as sA[r][c] = A(r,c) and
+// sB[j][k] = B'(j,k), so that out(i,j) = sum_k sA[i][k] sB[j][k].
+// On the FP64 matrix cores: 4 waves, each a 32x32 quarter
+// of the 64x64 tile as 2x2 blocks of v_mfma_f64_16x16x4 (A operand: lane l
+// holds A[l%16][l/16]; B operand: lane l holds B[l/16][l%16]; accumulator:
+// lane l, element r holds D[l/16 + 4r][l%16], as measured on gfx950 with a
+// one-wave probe).  Operands come from the
+// LDS-staged tiles, 4 doubles per lane per k-step of 4 instead of 8 LDS
+// reads per k for the 4x4 register tiling.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+struct MfAcc {
+  dbl4 d[2][2];
+};
+__device__ inline void mf_zero(MfAcc &m) {
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) m.d[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+}
+__device__ inline void tile_mm_mf(const double *A, const double *B, bool transB, double (*sA)[TS + 1],
+                                  double (*sB)[TS + 1], MfAcc &m) {
   __syncthreads();
   for (int t = threadIdx.x; t < TT; t += 256) {
-    const int r = t & 63, c = t >> 6;  // column-major element (r,c)
+    const int r = t & 63, c = t >> 6;
     sA[r][c] = A[t];
-    if (transB) sB[r][c] = B[t];       // B used as B'
-    else sB[c][r] = B[t];              // store B' so that the loop below is shared
+    if (transB) sB[r][c] = B[t];
+    else sB[c][r] = B[t];
   }
   __syncthreads();
-  for (int k = 0; k < TS; ++k) {
-    double a[4], b[4];
-    for (int i = 0; i < 4; ++i) a[i] = sA[4 * ty + i][k];
-    for (int j = 0; j < 4; ++j) b[j] = sB[4 * tx + j][k];
-    for (int i = 0; i < 4; ++i)
-      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1), li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < TS; k0 += 4) {
+    const double a0 = sA[r0 + li][k0 + lk], a1 = sA[r0 + 16 + li][k0 + lk];
+    const double b0 = sB[c0 + li][k0 + lk], b1 = sB[c0 + 16 + li][k0 + lk];
+    m.d[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, m.d[0][0], 0, 0, 0);
+    m.d[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, m.d[0][1], 0, 0, 0);
+    m.d[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, m.d[1][0], 0, 0, 0);
+    m.d[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, m.d[1][1], 0, 0, 0);
   }
+}
+// accumulator -> 4x4 per thread, acc[i][j] = out(4*(tid/16)+i, 4*(tid%16)+j), through sA
+__device__ inline void mf_to_acc(const MfAcc &m, double (*sA)[TS + 1], double acc[4][4]) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1), li = lane & 15, lq = lane >> 4;
+  __syncthreads();
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) sA[r0 + 16 * bi + lq + 4 * r][c0 + 16 * bj + li] = m.d[bi][bj][r];
+  __syncthreads();
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = sA[4 * ty + i][4 * tx + j];
 }
 
 // C_IJ -= sum_pairs L_IK L_JK'  (one workgroup per chunk of at most UCH pairs
@@ -129,9 +162,12 @@ __global__ __launch_bounds__(256) void k_update(const int32_t *chunk_tile, const
   __shared__ double sA[TS][TS + 1];
   __shared__ double sB[TS][TS + 1];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4] = {};
+  MfAcc m;
+  mf_zero(m);
   for (int64_t p = p0; p < p1; ++p)
-    tile_mm(tiles + (int64_t)pairs[2 * p] * TT, tiles + (int64_t)pairs[2 * p + 1] * TT, true, sA, sB, acc);
+    tile_mm_mf(tiles + (int64_t)pairs[2 * p] * TT, tiles + (int64_t)pairs[2 * p + 1] * TT, true, sA, sB, m);
+  double acc[4][4];
+  mf_to_acc(m, sA, acc);
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) atomicAdd(C + (4 * ty + i) + TS * (4 * tx + j), -acc[i][j]);
 }
@@ -209,8 +245,11 @@ __global__ __launch_bounds__(256) void k_trsm(const int32_t *tile, const int32_t
   double *C = tiles + (int64_t)tile[task] * TT;
   __shared__ double sA[TS][TS + 1];
   __shared__ double sB[TS][TS + 1];
-  double acc[4][4] = {};
-  tile_mm(C, dinv + (int64_t)col[task] * TT, true, sA, sB, acc);
+  MfAcc m;
+  mf_zero(m);
+  tile_mm_mf(C, dinv + (int64_t)col[task] * TT, true, sA, sB, m);
+  double acc[4][4];
+  mf_to_acc(m, sA, acc);
   __syncthreads();
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   double *CT = tilesT + (int64_t)tile[task] * TT;
@@ -243,9 +282,12 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
     }
     return;
   }
-  double acc[4][4] = {};
+  MfAcc m;
+  mf_zero(m);
   for (int64_t p = p0; p < p1; ++p)
-    tile_mm(tiles + (int64_t)pairs[2 * p] * TT, X + (int64_t)pairs[2 * p + 1] * TT, false, sA, sB, acc);
+    tile_mm_mf(tiles + (int64_t)pairs[2 * p] * TT, X + (int64_t)pairs[2 * p + 1] * TT, false, sA, sB, m);
+  double acc[4][4];
+  mf_to_acc(m, sA, acc);
   // S = acc (64x64) -> LDS, then Xo = -Di * S
   __syncthreads();
   for (int i = 0; i < 4; ++i)
