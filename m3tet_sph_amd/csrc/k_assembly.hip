@@ -15,9 +15,11 @@
 //   Fb = sum_ip w Fg r_hat N        (30)
 // It then applies the node rotations RR' . RR of flow3d_patera_mg_sph_p.m:
 // 233-243, lifts the prescribed velocities into bc_rhs (:280-283), removes
-// the fixed dofs (:297-298) and scatter-adds 3x3 blocks into the sliced
-// block-sparse operators.  The rotated, masked element matrix is kept for the
-// element-local Galerkin product of the next multigrid level.
+// the fixed dofs (:297-298) and scatter-adds the G, GT and M blocks into the
+// sliced block-sparse operators.  The rotated, masked element matrix Kel is
+// kept; the finest K is summed from it row by row (k_gatherK, no global
+// atomics) and the element-local Galerkin product of the next multigrid level
+// reads it too.
 #include "m3_internal.h"
 
 #include <cmath>
@@ -163,9 +165,11 @@ struct AsmArgs {
   double g, Bscale, Dens0, minV, maxV, C2;
   int hasMin, hasMax;
   SellView K, G, GT, M;
+  const int32_t *slots;  // ASM_SLOTS per element
   double *Fb, *bc_rhs, *Kel;
   double *dbgK, *dbgG, *dbgM, *dbgF;
   int write_global;
+  int scatterK;          // 0: K is summed afterwards from Kel by k_gatherK
   int *err;
 };
 
@@ -409,11 +413,11 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
       a.Kel[e * 465 + t] = s_Kr[pq & 255][pq >> 8];
     }
   }
-  // ---------------- scatter K blocks
-  for (int t = lane; t < 100; t += 64) {
+  // ---------------- scatter K blocks (unless the row-owner gather k_gatherK sums them)
+  const int32_t *slot = a.slots + e * ASM_SLOTS;  // checked at m3_set_mesh
+  for (int t = lane; t < (a.scatterK ? 100 : 0); t += 64) {
     const int na = t / 10, nb = t % 10;
-    const int64_t pos = sell_find(a.K.slice_ptr, a.K.rowlen, a.K.col, s_nod[na], s_nod[nb]);
-    if (pos < 0) { atomicOr(a.err, 8); continue; }
+    const int64_t pos = slot[t];
     for (int cc = 0; cc < 3; ++cc)
       for (int dd = 0; dd < 3; ++dd)
         atomicAdd(a.K.val + (3 * cc + dd) * a.K.nslot + pos, s_Kr[3 * na + cc][3 * nb + dd]);
@@ -427,21 +431,13 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
       for (int i = 0; i < 3; ++i) acc += s_RR[na][3 * i + cc] * s_Gr[3 * na + i][v];
       gr[cc] = acc;
     }
-    const int node = s_nod[na], vert = s_nod[v];
-    const int64_t p1 = sell_find(a.G.slice_ptr, a.G.rowlen, a.G.col, node, vert);
-    const int64_t p2 = sell_find(a.GT.slice_ptr, a.GT.rowlen, a.GT.col, vert, node);
-    if (p1 < 0 || p2 < 0) atomicOr(a.err, 8);
-    else
-      for (int cc = 0; cc < 3; ++cc) {
-        atomicAdd(a.G.val + cc * a.G.nslot + p1, gr[cc]);
-        atomicAdd(a.GT.val + cc * a.GT.nslot + p2, gr[cc]);
-      }
+    const int64_t p1 = slot[100 + lane], p2 = slot[140 + lane];
+    for (int cc = 0; cc < 3; ++cc) {
+      atomicAdd(a.G.val + cc * a.G.nslot + p1, gr[cc]);
+      atomicAdd(a.GT.val + cc * a.GT.nslot + p2, gr[cc]);
+    }
   }
-  if (lane >= 32 && lane < 48) {
-    const int64_t p = sell_find(a.M.slice_ptr, a.M.rowlen, a.M.col, s_nod[mv], s_nod[mw]);
-    if (p < 0) atomicOr(a.err, 8);
-    else atomicAdd(a.M.val + p, mval);
-  }
+  if (lane >= 32 && lane < 48) atomicAdd(a.M.val + slot[180 + (lane - 32)], mval);
   // ---------------- Fb (rotated)
   if (lane < 30) {
     const int na = lane / 3, cc = lane % 3;
@@ -534,6 +530,52 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
   }
 }
 
+// Finest K summed row by row from the element matrices Kel, without global
+// atomics: one wavefront per node row i accumulates its blocks in LDS.  Lane
+// work item = (incident element e with i at local node a, local column node
+// b); the 3x3 block (a,b) of Kel lands in stored block k of row i, where the
+// slot of (a,b) comes from the per-element slot map.  Kel is the packed lower
+// (column-major) triangle of the symmetric 30x30 rotated, masked matrix.
+// Every stored block of the row is written, so K needs no clearing first.
+constexpr int GK_ROWS = 4;      // rows (wavefronts) per block
+constexpr int GK_MAXLEN = 128;  // stored blocks per row (checked at m3_set_mesh)
+__global__ __launch_bounds__(64 * GK_ROWS) void k_gatherK(SellView K, int64_t nrow, const int64_t *inc_ptr,
+                                                         const int32_t *inc, const int32_t *slots,
+                                                         const double *Kel) {
+  __shared__ double acc[GK_ROWS][GK_MAXLEN * 9];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * GK_ROWS + w;
+  const bool live = i < nrow;
+  const int len = live ? K.rowlen[i] : 0;
+  const int64_t base = live ? K.slice_ptr[i >> 6] + (i & 63) : 0;
+  double *A = acc[w];
+  for (int t = lane; t < len * 9; t += 64) A[t] = 0.0;
+  __syncthreads();
+  if (live) {
+    const int64_t t0 = inc_ptr[i] * 10, t1 = inc_ptr[i + 1] * 10;
+    for (int64_t t = t0 + lane; t < t1; t += 64) {
+      const int64_t j = t / 10;
+      const int b = (int)(t - 10 * j);
+      const int ea = inc[j];
+      const int64_t e = ea / 10;
+      const int na = ea - 10 * (int)e;
+      const int k = (int)((slots[e * ASM_SLOTS + 10 * na + b] - base) >> 6);
+      const double *ke = Kel + e * 465;
+      for (int cc = 0; cc < 3; ++cc)
+        for (int dd = 0; dd < 3; ++dd) {
+          const int p = 3 * na + cc, q = 3 * b + dd;
+          const int lo = p < q ? p : q, hi = p < q ? q : p;
+          atomicAdd(&A[9 * k + 3 * cc + dd], ke[lo * 30 - lo * (lo - 1) / 2 + (hi - lo)]);
+        }
+    }
+  }
+  __syncthreads();
+  for (int t = lane; t < len * 9; t += 64) {
+    const int k = t / 9, v = t - 9 * k;
+    K.val[v * K.nslot + base + 64 * (int64_t)k] = A[t];
+  }
+}
+
 // identity at fixed dofs on the diagonal blocks, inverse diagonal
 __global__ void k_fix_diag(SellView K, const double *mask, int64_t nnod, double *dinv) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -614,6 +656,7 @@ void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double
   a.hasMax = c->phys.has_maxVisc;
   a.C2 = c->phys.type_Dmat == 0 ? -2.0 / 3.0 : 0.0;
   a.K = f.K.view();
+  a.slots = c->asm_slots.p;
   a.G = c->G.view();
   a.GT = c->GT.view();
   a.M = c->M.view();
@@ -625,6 +668,7 @@ void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double
   a.dbgM = dbgM;
   a.dbgF = dbgF;
   a.write_global = write_global ? 1 : 0;
+  a.scatterK = c->gather_K ? 0 : 1;
   a.err = c->errflag.p;
   hipLaunchKernelGGL(k_assemble, dim3((unsigned)f.nel), dim3(64), 0, c->stream, a);
   HIPCHECK(hipGetLastError());
@@ -644,6 +688,13 @@ void launch_rap(m3_ctx *c, int l) {
   a.K = cl.K.view();
   a.err = c->errflag.p;
   hipLaunchKernelGGL(k_rap, dim3((unsigned)cl.nel), dim3(64), 0, c->stream, a);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_gather_K(m3_ctx *c) {
+  LevelDev &f = c->L[0];
+  hipLaunchKernelGGL(k_gatherK, dim3((unsigned)((f.nnod + GK_ROWS - 1) / GK_ROWS)), dim3(64 * GK_ROWS), 0,
+                     c->stream, f.K.view(), f.nnod, c->inc_ptr.p, c->inc.p, c->asm_slots.p, f.Kel.p);
   HIPCHECK(hipGetLastError());
 }
 

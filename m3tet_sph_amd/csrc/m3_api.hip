@@ -77,6 +77,20 @@ void adjacency(const std::vector<int32_t> &el2nod, int64_t nel, int64_t nrow, in
   for (int64_t n = 0; n < nrow; ++n) std::copy(rows[n].begin(), rows[n].end(), cols.begin() + ptr[n]);
 }
 
+// slot of block (row, col) in a SELL pattern, -1 if absent
+int32_t host_find(const Sell &S, int64_t row, int64_t col) {
+  const int64_t base = S.h_slice_ptr[row >> 6] + (row & 63);
+  int lo = 0, hi = S.h_rowlen[row] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t cv = S.h_col[base + (int64_t)mid * 64];
+    if (cv == col) return (int32_t)(base + (int64_t)mid * 64);
+    if (cv < col) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
 const int ALL10[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
 const int V4[4] = {0, 1, 2, 3};
 
@@ -231,6 +245,41 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       c->GT.build(f.nV, f.nnod, ptr, cols, 3);
       adjacency(f.h_el2nod, f.nel, f.nV, 4, V4, 4, V4, ptr, cols);
       c->M.build(f.nV, f.nV, ptr, cols, 1);
+      // slot of every element block in K (10x10), G (10x4), GT (4x10), M (4x4):
+      // the assembly scatters without searching the sparse rows
+      std::vector<int32_t> slots((size_t)f.nel * ASM_SLOTS);
+      bool missing = false;
+#pragma omp parallel for schedule(static) reduction(|| : missing)
+      for (int64_t e = 0; e < f.nel; ++e) {
+        const int32_t *n = &f.h_el2nod[10 * e];
+        int32_t *o = &slots[(size_t)e * ASM_SLOTS];
+        for (int a = 0; a < 10; ++a)
+          for (int b = 0; b < 10; ++b) o[10 * a + b] = host_find(f.K, n[a], n[b]);
+        for (int a = 0; a < 10; ++a)
+          for (int v = 0; v < 4; ++v) {
+            o[100 + 4 * a + v] = host_find(c->G, n[a], n[v]);
+            o[140 + 4 * a + v] = host_find(c->GT, n[v], n[a]);
+          }
+        for (int v = 0; v < 4; ++v)
+          for (int w = 0; w < 4; ++w) o[180 + 4 * v + w] = host_find(c->M, n[v], n[w]);
+        for (int k = 0; k < ASM_SLOTS; ++k) missing = missing || o[k] < 0;
+      }
+      M3_REQUIRE(!missing, M3_E_MESH, "element block missing from the sparse pattern");
+      c->asm_slots.upload(slots);
+      // node -> (element, local node) incidence for the row-owner assembly of K
+      std::vector<int64_t> iptr(f.nnod + 1, 0);
+      for (int64_t k = 0; k < 10 * f.nel; ++k) ++iptr[f.h_el2nod[k] + 1];
+      for (int64_t n = 0; n < f.nnod; ++n) iptr[n + 1] += iptr[n];
+      std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+      std::vector<int32_t> inc(10 * f.nel);
+      for (int64_t e = 0; e < f.nel; ++e)
+        for (int a = 0; a < 10; ++a) inc[fill[f.h_el2nod[10 * e + a]]++] = (int32_t)(10 * e + a);
+      c->inc_ptr.upload(iptr);
+      c->inc.upload(inc);
+      int maxlen = 0;
+      for (int64_t r = 0; r < f.nnod; ++r) maxlen = std::max(maxlen, f.K.h_rowlen[r]);
+      c->gather_K = maxlen <= 128 && 10 * f.nel < (int64_t)INT32_MAX;
+      f.K.val.zero(c->stream);  // padding slots stay zero
     }
     // coordinates and nodal frames (make_rotation_matrix.m:69-110)
     std::vector<double> X(3 * f.nnod), S(3 * f.nnod), fr(9 * f.nnod);
@@ -400,7 +449,7 @@ static void do_assemble(m3_ctx *c, const double *Visc, const double *Dens, const
   } else {
     M3_REQUIRE(c->have_fields, M3_E_STATE, "Visc = Dens = NULL needs m3_rock_prop first");
   }
-  f.K.val.zero(s);
+  if (!c->gather_K) f.K.val.zero(s);  // the row-owner gather writes every stored block
   c->G.val.zero(s);
   c->GT.val.zero(s);
   c->M.val.zero(s);
@@ -408,6 +457,7 @@ static void do_assemble(m3_ctx *c, const double *Visc, const double *Dens, const
   c->bc_rhs.zero(s);
   c->errflag.zero(s);
   m3::launch_assemble(c, nullptr, nullptr, nullptr, nullptr, true);
+  if (c->gather_K) m3::launch_gather_K(c);
   m3::launch_fix_diag(c, 0);
   m3::launch_lumped_mm(c);
   raise_flags(check_err(c));

@@ -138,6 +138,10 @@ struct CholTiles {
   DBuf<double> w, y, t, x;       // T*64 solve vectors
 };
 
+// sparse slots of one element's blocks: K 10x10 at [10a+b], G 10x4 at
+// [100+4a+v], GT 4x10 at [140+4a+v], M 4x4 at [180+4v+w]
+constexpr int ASM_SLOTS = 196;
+
 struct LevelDev {
   int64_t nel = 0, nnod = 0, nV = 0;
   std::vector<int32_t> h_el2nod;   // 0-based nel*10
@@ -176,6 +180,12 @@ struct m3_ctx {
   Sell G, GT, M;                   // GG (3x1), GG' (1x3), MM (scalar)
   DBuf<double> Fb, bc_rhs, U0, LiMM;
   DBuf<int32_t> errflag;
+  DBuf<int32_t> asm_slots;         // per finest element: ASM_SLOTS sparse slots (see k_assemble)
+  // row-owner assembly of the finest K from the element matrices: for each
+  // node the (element, local node) pairs it belongs to, as 10*element + local
+  DBuf<int64_t> inc_ptr;
+  DBuf<int32_t> inc;
+  bool gather_K = false;
   std::vector<int64_t> h_ifix;
   std::vector<double> h_vfix;
   bool have_mesh = false, have_bc = false, assembled = false, restricted = false,
@@ -235,6 +245,7 @@ void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double
                      bool write_global);
 void launch_rap(m3_ctx *c, int l);  // level l element matrices -> level l+1
 void launch_fix_diag(m3_ctx *c, int l);
+void launch_gather_K(m3_ctx *c);  // finest K from the element matrices (no atomics)
 void launch_diag_inv(m3_ctx *c, int l);
 void launch_lumped_mm(m3_ctx *c);
 
