@@ -13,13 +13,16 @@
 //   potrf   L_JJ = chol(C_JJ) and inv(L_JJ)  (one workgroup per column)
 //   trsm    L_IJ = C_IJ inv(L_JJ)'           (one workgroup per tile)
 // followed by the explicit inverse X_S = inv(L_SS) of each block's diagonal
-// part.  The triangular solves then take four launches per tree height:
-//   forward:  y_S = X_S w_S ;  w_A -= L_AS y_S       (ancestors A)
-//   backward: t_S = y_S - L_AS' x_A ;  x_S = X_S' t_S
-// each a batch of independent 64-row gathers.
+// part and the products Z_AS = L_AS X_S with the rows A below the block.  The
+// triangular solves then take one launch per tree height and direction:
+//   forward:  y_S = X_S w_S  and  w_A -= Z_AS w_S     (ancestors A)
+//   backward: x_S = X_S' y_S - Z_AS' x_A
+// (Z_AS w_S = L_AS y_S and X_S'(y_S - L_AS' x_A) = X_S' y_S - Z_AS' x_A), each
+// a batch of independent 64-row gathers.
 #include "m3_internal.h"
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <numeric>
 
@@ -310,43 +313,59 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
     }
 }
 
-// out[row] += sign * sum_p M_p v[vcol_p]   (64-row gathers; one workgroup per
-// chunk of at most GCH items of a task, partial sums added atomically)
-constexpr int GCH = 1;  // 4 / 2 / 1 items per workgroup: 1 fastest (latency-bound)
-__global__ __launch_bounds__(256) void k_gtask(int chunk0, const int32_t *crow, const int64_t *cbeg,
-                                               const int64_t *cend, const int32_t *mat,
-                                               const int32_t *vcol, const double *mats, const double *v,
-                                               double sign, double *out) {
-  const int ck = chunk0 + blockIdx.x;
-  const int r = crow[ck];
+// Z_Ij = sum_k L_Ik X_kj over the block's columns k >= j, and its transposed copy
+__global__ __launch_bounds__(256) void k_zprod(const int32_t *z_tile, const int64_t *pair_ptr,
+                                               const int32_t *pairs, const double *tiles, double *X,
+                                               double *XT) {
+  const int task = blockIdx.x;
+  const int64_t p0 = pair_ptr[task], p1 = pair_ptr[task + 1];
+  __shared__ double sA[TS][TS + 1];
+  __shared__ double sB[TS][TS + 1];
+  MfAcc m;
+  mf_zero(m);
+  for (int64_t p = p0; p < p1; ++p)
+    tile_mm_mf(tiles + (int64_t)pairs[2 * p] * TT, X + (int64_t)pairs[2 * p + 1] * TT, false, sA, sB, m);
+  double acc[4][4];
+  mf_to_acc(m, sA, acc);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double *Zo = X + (int64_t)z_tile[task] * TT;
+  double *ZTo = XT + (int64_t)z_tile[task] * TT;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * ty + i, c = 4 * tx + j;
+      Zo[r + TS * c] = acc[i][j];
+      ZTo[c + TS * r] = acc[i][j];
+    }
+}
+
+// One 64x64 tile times a 64-vector per workgroup: item (row, mat, vcol, sel)
+// adds M[mat] in0[vcol] to out0[row] (sel 0) or subtracts M[mat] in1[vcol]
+// from out1[row] (sel 1); items of one row are summed atomically.
+__global__ __launch_bounds__(256) void k_gtask(const CholTiles::Item *items, int item0, const double *mats,
+                                               const double *in0, const double *in1, double *out0,
+                                               double *out1) {
+  const CholTiles::Item it = items[item0 + blockIdx.x];
   const int i = threadIdx.x & 63, part = threadIdx.x >> 6;  // 4 waves split the columns
   __shared__ double red[4][TS];
-  double acc = 0;
-  for (int64_t p = cbeg[ck]; p < cend[ck]; ++p) {
-    const double *M = mats + (int64_t)mat[p] * TT;
-    const double *vv = v + (int64_t)vcol[p] * TS;
-    // all 16 loads of this thread's columns in flight before the FMAs: the
-    // phases are latency-bound, not bandwidth-bound
-    double mv[TS / 4], xv[TS / 4];
+  const double *M = mats + (int64_t)it.mat * TT;
+  const double *vv = (it.sel ? in1 : in0) + (int64_t)it.vcol * TS;
+  // all 16 loads of this thread's columns in flight before the FMAs: the
+  // phases are latency-bound, not bandwidth-bound
+  double mv[TS / 4], xv[TS / 4];
 #pragma unroll
-    for (int j = 0; j < TS / 4; ++j) {
-      mv[j] = M[i + TS * (part + 4 * j)];
-      xv[j] = vv[part + 4 * j];
-    }
-#pragma unroll
-    for (int j = 0; j < TS / 4; ++j) acc += mv[j] * xv[j];
+  for (int j = 0; j < TS / 4; ++j) {
+    mv[j] = M[i + TS * (part + 4 * j)];
+    xv[j] = vv[part + 4 * j];
   }
+  double acc = 0;
+#pragma unroll
+  for (int j = 0; j < TS / 4; ++j) acc += mv[j] * xv[j];
   red[part][i] = acc;
   __syncthreads();
   if (part == 0) {
     const double s = red[0][i] + red[1][i] + red[2][i] + red[3][i];
-    atomicAdd(out + (int64_t)r * TS + i, sign * s);
+    atomicAdd((it.sel ? out1 : out0) + (int64_t)it.row * TS + i, it.sel ? -s : s);
   }
-}
-
-__global__ void k_copyv(int64_t n, const double *x, double *y) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) y[i] = x[i];
 }
 
 __global__ void k_to_tiles(int64_t npos, const int32_t *pos2dof, const double *x, double *w) {
@@ -372,35 +391,19 @@ inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) /
 
 template <class T>
 void up(DBuf<T> &b, const std::vector<T> &h) {
-  if (h.empty()) b.upload(std::vector<T>{T(0)});
+  if (h.empty()) b.upload(std::vector<T>{T{}});
   else b.upload(h);
 }
 
-// a gather phase built per height; every task is cut into chunks of at most
-// GCH items, one workgroup each
+// gather items of one direction, grouped by height
 struct PhaseBuilder {
-  std::vector<int32_t> crow, mat, vcol, hptr{0};
-  std::vector<int64_t> cbeg, cend;
-  int cur = -1;
-  int64_t start = 0;
-  void task(int r) { cur = r; start = (int64_t)mat.size(); }
-  void item(int m, int c) { mat.push_back(m); vcol.push_back(c); }
-  void end_task() {
-    const int64_t n = (int64_t)mat.size();
-    for (int64_t b = start; b < n; b += GCH) {
-      crow.push_back(cur);
-      cbeg.push_back(b);
-      cend.push_back(std::min<int64_t>(b + GCH, n));
-    }
-  }
-  void end_height() { hptr.push_back((int32_t)crow.size()); }
+  std::vector<CholTiles::Item> items;
+  std::vector<int32_t> hptr{0};
+  void item(int row, int mat, int vcol, int sel) { items.push_back({row, mat, vcol, sel}); }
+  void end_height() { hptr.push_back((int32_t)items.size()); }
   void upload(CholTiles::Phase &p) {
     p.h_ptr = hptr;
-    up(p.row, crow);
-    up(p.mat, mat);
-    up(p.vcol, vcol);
-    up(p.tptr, cbeg);
-    up(p.tend, cend);
+    up(p.items, items);
   }
 };
 
@@ -600,54 +603,57 @@ void chol_analyse(m3_ctx *c) {
     }
     ch.h_inv_ptr[d + 1] = (int32_t)inv_x.size();
   }
-  // solve phases
+  // Z tiles: for each block S and row tile I below it, Z_Ij = sum_k L_Ik X_kj
+  // (k >= j in S); stored after the X tiles in the same pool
+  std::vector<int32_t> z_tile, z_pairs;
+  std::vector<int64_t> z_pair_ptr{0};
+  std::vector<std::vector<std::array<int32_t, 3>>> zlist(ch.nblock);  // (I, j, pool index)
+  for (int b = 0; b < ch.nblock; ++b) {
+    std::vector<int32_t> below;
+    for (int k = blk_t0[b]; k < blk_t1[b]; ++k)
+      for (int32_t t = ch.h_colptr[k]; t < ch.h_colptr[k + 1]; ++t)
+        if (ch.h_rowtile[t] >= blk_t1[b]) below.push_back(ch.h_rowtile[t]);
+    std::sort(below.begin(), below.end());
+    below.erase(std::unique(below.begin(), below.end()), below.end());
+    for (int j = blk_t0[b]; j < blk_t1[b]; ++j)
+      for (int I : below) {
+        const size_t before = z_pairs.size();
+        for (int k = j; k < blk_t1[b]; ++k) {
+          const int32_t tl = tile_of(I, k);
+          if (tl >= 0) {
+            z_pairs.push_back(tl);
+            z_pairs.push_back(xidx(b, k, j));
+          }
+        }
+        if (z_pairs.size() == before) continue;
+        const int32_t zi = (int32_t)(ch.nX + (int64_t)z_tile.size());
+        z_tile.push_back(zi);
+        z_pair_ptr.push_back((int64_t)z_pairs.size() / 2);
+        zlist[b].push_back({I, j, zi});
+      }
+  }
+  ch.nZ = (int64_t)z_tile.size();
+  // solve phases, one launch per height and direction:
+  //   forward  y_S = X_S w_S  and  w_A -= Z_AS w_S   (both read the final w_S)
+  //   backward x_S = X_S' y_S - Z_AS' x_A            (x_A final from the heights above)
   std::vector<std::vector<int>> blocks_at(H);
   for (int b = 0; b < ch.nblock; ++b) blocks_at[hb[b]].push_back(b);
-  PhaseBuilder fd, fu, bu, bd;
+  PhaseBuilder fw, bw;
   for (int h = 0; h < H; ++h) {
-    // forward diagonal: y_i = sum_{j<=i in block} X_ij w_j
-    for (int b : blocks_at[h])
-      for (int i = blk_t0[b]; i < blk_t1[b]; ++i) {
-        fd.task(i);
-        for (int j = blk_t0[b]; j <= i; ++j) fd.item(xidx(b, i, j), j);
-        fd.end_task();
-      }
-    fd.end_height();
-    // forward update: w_I -= sum L_Ij y_j for j in blocks of height h
-    std::map<int, std::vector<std::pair<int32_t, int32_t>>> tgt;
-    for (int b : blocks_at[h])
-      for (int j = blk_t0[b]; j < blk_t1[b]; ++j)
-        for (int32_t t = ch.h_colptr[j]; t < ch.h_colptr[j + 1]; ++t) {
-          const int I = ch.h_rowtile[t];
-          if (I >= blk_t1[b]) tgt[I].push_back({t, j});
-        }
-    for (auto &kv : tgt) {
-      fu.task(kv.first);
-      for (auto &tj : kv.second) fu.item(tj.first, tj.second);
-      fu.end_task();
+    for (int b : blocks_at[h]) {
+      for (int i = blk_t0[b]; i < blk_t1[b]; ++i)
+        for (int j = blk_t0[b]; j <= i; ++j) fw.item(i, xidx(b, i, j), j, 0);
+      for (auto &z : zlist[b]) fw.item(z[0], z[2], z[1], 1);
     }
-    fu.end_height();
+    fw.end_height();
   }
   for (int h = H - 1; h >= 0; --h) {
-    // backward update: t_j = y_j - sum_{I outside block} L_Ij' x_I
-    for (int b : blocks_at[h])
-      for (int j = blk_t0[b]; j < blk_t1[b]; ++j) {
-        bu.task(j);
-        for (int32_t t = ch.h_colptr[j]; t < ch.h_colptr[j + 1]; ++t) {
-          const int I = ch.h_rowtile[t];
-          if (I >= blk_t1[b]) bu.item(t, I);
-        }
-        bu.end_task();
-      }
-    bu.end_height();
-    // backward diagonal: x_j = sum_{i>=j in block} X_ij' t_i
-    for (int b : blocks_at[h])
-      for (int j = blk_t0[b]; j < blk_t1[b]; ++j) {
-        bd.task(j);
-        for (int i = j; i < blk_t1[b]; ++i) bd.item(xidx(b, i, j), i);
-        bd.end_task();
-      }
-    bd.end_height();
+    for (int b : blocks_at[h]) {
+      for (int j = blk_t0[b]; j < blk_t1[b]; ++j)
+        for (int i = j; i < blk_t1[b]; ++i) bw.item(j, xidx(b, i, j), i, 0);
+      for (auto &z : zlist[b]) bw.item(z[1], z[2], z[0], 1);
+    }
+    bw.end_height();
   }
   // gather map from the coarse operator values (SoA, 9*nslot)
   std::vector<int64_t> gsrc, gdst;
@@ -686,17 +692,18 @@ void chol_analyse(m3_ctx *c) {
   up(ch.diagtile, diagt);
   up(ch.gsrc, gsrc);
   up(ch.gdst, gdst);
-  fd.upload(ch.fdiag);
-  fu.upload(ch.fupd);
-  bu.upload(ch.bupd);
-  bd.upload(ch.bdiag);
+  up(ch.z_tile, z_tile);
+  up(ch.z_pairs, z_pairs);
+  up(ch.z_pair_ptr, z_pair_ptr);
+  fw.upload(ch.fwd);
+  bw.upload(ch.bwd);
   ch.tiles.alloc((size_t)ch.ntile * TT);
   ch.tilesT.alloc((size_t)ch.ntile * TT);
   ch.dinv.alloc((size_t)T * TT);
   ch.dinvT.alloc((size_t)T * TT);
-  ch.X.alloc((size_t)ch.nX * TT);
-  ch.XT.alloc((size_t)ch.nX * TT);
-  for (auto *b : {&ch.w, &ch.y, &ch.t, &ch.x}) b->alloc((size_t)T * TS);
+  ch.X.alloc((size_t)(ch.nX + ch.nZ) * TT);
+  ch.XT.alloc((size_t)(ch.nX + ch.nZ) * TT);
+  for (auto *b : {&ch.w, &ch.y, &ch.x}) b->alloc((size_t)T * TS);
   c->factored = false;
 }
 
@@ -727,16 +734,18 @@ void chol_factor(m3_ctx *c) {
       hipLaunchKernelGGL(k_inv, dim3(b - a), dim3(256), 0, s, ch.inv_x.p, ch.inv_row.p, ch.inv_pair_ptr.p,
                          ch.inv_pairs.p, a, ch.tiles.p, ch.dinv.p, ch.X.p, ch.XT.p);
   }
+  if (ch.nZ > 0)
+    hipLaunchKernelGGL(k_zprod, dim3((unsigned)ch.nZ), dim3(256), 0, s, ch.z_tile.p, ch.z_pair_ptr.p,
+                       ch.z_pairs.p, ch.tiles.p, ch.X.p, ch.XT.p);
   HIPCHECK(hipGetLastError());
   c->factored = true;
 }
 
-static void run_phase(hipStream_t s, CholTiles::Phase &p, int h, const double *mats, const double *v,
-                      double sign, double *out) {
+static void run_phase(hipStream_t s, CholTiles::Phase &p, int h, const double *mats, const double *in0,
+                      const double *in1, double *out0, double *out1) {
   const int a = p.h_ptr[h], b = p.h_ptr[h + 1];
   if (b > a)
-    hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, a, p.row.p, p.tptr.p, p.tend.p, p.mat.p,
-                       p.vcol.p, mats, v, sign, out);
+    hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, p.items.p, a, mats, in0, in1, out0, out1);
 }
 
 void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
@@ -747,17 +756,8 @@ void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
   hipLaunchKernelGGL(k_to_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.w.p);
   hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.y.p);
   hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.x.p);
-  // forward: y_S = X_S w_S ; w_A -= L_AS y_S
-  for (int h = 0; h < ch.H; ++h) {
-    run_phase(s, ch.fdiag, h, ch.X.p, ch.w.p, 1.0, ch.y.p);
-    run_phase(s, ch.fupd, h, ch.tiles.p, ch.y.p, -1.0, ch.w.p);
-  }
-  // backward: t_S = y_S - L_AS' x_A ; x_S = X_S' t_S
-  hipLaunchKernelGGL(k_copyv, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.y.p, ch.t.p);
-  for (int k = 0; k < ch.H; ++k) {
-    run_phase(s, ch.bupd, k, ch.tilesT.p, ch.x.p, -1.0, ch.t.p);
-    run_phase(s, ch.bdiag, k, ch.XT.p, ch.t.p, 1.0, ch.x.p);
-  }
+  for (int h = 0; h < ch.H; ++h) run_phase(s, ch.fwd, h, ch.X.p, ch.w.p, ch.w.p, ch.y.p, ch.w.p);
+  for (int k = 0; k < ch.H; ++k) run_phase(s, ch.bwd, k, ch.XT.p, ch.y.p, ch.x.p, ch.x.p, ch.x.p);
   const int64_t n3 = 3 * cl.nnod;
   hipLaunchKernelGGL(k_zero, dim3(nblk(n3)), dim3(256), 0, s, n3, x);
   hipLaunchKernelGGL(k_from_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, ch.x.p, x);

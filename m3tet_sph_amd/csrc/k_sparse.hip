@@ -604,21 +604,20 @@ static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
     VecSet<VT> v = vs(lv);
     VecSet<VT> vc = vs(cl);
     const double *wu = l == 0 ? WUP1 : WUP;
+    // prolongated correction into dn; three sweeps dn -> d -> dn -> d leave
+    // the level's result in d
     if constexpr (std::is_same<VT, double>::value)
       hipLaunchKernelGGL((k_prolong<double, double>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.nnod,
-                         lv.P_ptr.p, lv.P_col.p, lv.P_val.p, lv.mask.p, vc.d, v.dd, v.d);
+                         lv.P_ptr.p, lv.P_col.p, lv.P_val.p, lv.mask.p, vc.d, v.dd, v.dn);
     else
       hipLaunchKernelGGL((k_prolong<float, float>), dim3(nblk(lv.nnod)), dim3(256), 0, s, lv.nnod,
-                         lv.P_ptr.p, lv.P_col.p, lv.P_val32.p, lv.mask.p, vc.d, v.dd, v.d);
+                         lv.P_ptr.p, lv.P_col.p, lv.P_val32.p, lv.mask.p, vc.d, v.dd, v.dn);
     const int pk = std::is_same<VT, double>::value ? 0 : 1;
     pick_K(l, [&](auto K) {
-      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[0], (VT *)nullptr, v.dn); }
-      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[1], (VT *)nullptr, v.d); }
-      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[2], (VT *)nullptr, v.dn); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[0], (VT *)nullptr, v.d); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.d, v.r0, v.dinv, wu[1], (VT *)nullptr, v.dn); }
+      { ProfScope ps(c, pk, l == 0); sweep(s, K, lv.nnod, v.dn, v.r0, v.dinv, wu[2], (VT *)nullptr, v.d); }
     });
-    // result of the last sweep is in dn: swap so that d holds it
-    if constexpr (std::is_same<VT, double>::value) std::swap(lv.d, lv.dn);
-    else std::swap(lv.df, lv.dnf);
   }
 }
 
@@ -644,32 +643,35 @@ void vcycle(m3_ctx *c, const double *r, double *d, int precision) {
   HIPCHECK(hipGetLastError());
 }
 
+
 // FP32 / BF16 copies of the operators for the mixed-precision V-cycle
 void build_mixed(m3_ctx *c) {
   hipStream_t s = c->stream;
+  auto fit = [&](auto &b, size_t n) {
+    if (b.n != n) b.alloc(n);
+  };
   for (int l = 0; l < c->nmg; ++l) {
     LevelDev &lv = c->L[l];
     const int64_t nv = 9 * lv.K.nslot;
     const int64_t n3 = 3 * lv.nnod;
     if (l < c->nmg - 1) {
       if (l == 0) {
-        if (lv.K.val32.n != (size_t)nv) lv.K.val32.alloc(nv);
+        fit(lv.K.val32, nv);
         hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(nv)), dim3(256), 0, s, nv, lv.K.val.p,
                            lv.K.val32.p);
       } else {
-        if (lv.K.val16.n != (size_t)nv) lv.K.val16.alloc(nv);
+        fit(lv.K.val16, nv);
         hipLaunchKernelGGL(k_to_bf16, dim3(nblk(nv)), dim3(256), 0, s, nv, lv.K.val.p, lv.K.val16.p);
       }
     }
-    if (lv.dinv32.n != (size_t)n3) lv.dinv32.alloc(n3);
+    fit(lv.dinv32, n3);
     hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, lv.dinv.p,
                        lv.dinv32.p);
-    auto ensure = [&](DBuf<float> &b) { if (b.n != (size_t)n3) b.alloc(n3); };
-    ensure(lv.r0f); ensure(lv.r1f); ensure(lv.df); ensure(lv.dnf); ensure(lv.ddf); ensure(lv.rrf);
+    for (auto *b : {&lv.r0f, &lv.r1f, &lv.df, &lv.dnf, &lv.ddf, &lv.rrf}) fit(*b, n3);
     if (l < c->nmg - 1) {
       const int64_t np = lv.P_val.n, nr = lv.R_val.n;
-      if (lv.P_val32.n != (size_t)np) lv.P_val32.alloc(np);
-      if (lv.R_val32.n != (size_t)nr) lv.R_val32.alloc(nr);
+      fit(lv.P_val32, np);
+      fit(lv.R_val32, nr);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(np)), dim3(256), 0, s, np, lv.P_val.p, lv.P_val32.p);
       hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(nr)), dim3(256), 0, s, nr, lv.R_val.p, lv.R_val32.p);
     }

@@ -123,19 +123,26 @@ struct CholTiles {
   DBuf<int64_t> upd_pair_ptr, upd_pair_end;  // per update chunk: pair range
   // supernodes: explicit inverse of each block's diagonal part
   int nblock = 0, H = 0, maxm = 0;
-  int64_t nX = 0;
-  DBuf<double> X, XT;            // inverse tiles (block-local m x m, column-major tiles)
+  int64_t nX = 0, nZ = 0;
+  // tile pool X | Z and its transposed copies: X = inverse of each block's
+  // diagonal part (block-local m x m, column-major tiles); Z_Ij = L_Ij-part
+  // times X for the rows I below a block (Z_Ij = sum_k L_Ik X_kj)
+  DBuf<double> X, XT;
   std::vector<int32_t> h_inv_ptr;            // per step d: task range
   DBuf<int32_t> inv_x, inv_row;              // target X tile, row tile (for dinv)
   DBuf<int64_t> inv_pair_ptr;
   DBuf<int32_t> inv_pairs;                   // (L tile, X tile) pairs
-  // solves: four gather-task phases per height
+  DBuf<int32_t> z_tile, z_pairs;             // Z target (pool index); (L tile, X tile) pairs
+  DBuf<int64_t> z_pair_ptr;
+  // solves: one gather phase per height and direction
+  struct Item {                              // out_sel[row] +-= M[mat] in_sel[vcol]
+    int32_t row, mat, vcol, sel;
+  };
   struct Phase {
-    std::vector<int32_t> h_ptr;  // per height: chunk range
-    DBuf<int32_t> row, mat, vcol;
-    DBuf<int64_t> tptr, tend;    // chunk item ranges
-  } fdiag, fupd, bupd, bdiag;
-  DBuf<double> w, y, t, x;       // T*64 solve vectors
+    std::vector<int32_t> h_ptr;  // per height: item range
+    DBuf<Item> items;
+  } fwd, bwd;
+  DBuf<double> w, y, x;          // T*64 solve vectors
 };
 
 // sparse slots of one element's blocks: K 10x10 at [10a+b], G 10x4 at
