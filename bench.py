@@ -48,8 +48,8 @@ HBM_PEAK_GBS = 8000.0
 # plain copy of known size); recorded in profiles/c1_k_smooth_hbm_traffic.md.
 # Keyed by (config, kernel); other configs report null.
 MEASURED_TRAFFIC = {
-    ("c1", "k_smooth<float,float,6,1,2,true>"): 520891157,
-    ("c1", "k_smooth<double,double,6,1,2,true>"): 994093278,
+    ("c1", "k_smooth<float,float,6,1,2,true>"): 520874482,
+    ("c1", "k_smooth<double,double,6,1,2,true>"): 994168545,
 }
 
 
