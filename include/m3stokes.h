@@ -249,6 +249,49 @@ int m3_get_operator(m3_ctx *ctx, int32_t level, int64_t *rowptr, int32_t *col, d
 int m3_sl_advect(m3_ctx *ctx, const double *T, const double *U, double dt, double T_cmb, double T_surf,
                  double *T_out);
 
+/* Options of the thermal diffusion step, SETTINGS.OPTS_D
+ * (default_numerics_M3TET.m:95-134; defaults in brackets). */
+typedef struct {
+  double theta;        /* OPTS_D.theta: time rule, 1 backward Euler [1], 0.5 Crank-Nicolson, 0 explicit */
+  int32_t lumping;     /* OPTS_D.lumping: 1 'yes' (row-sum mass matrix) [1], 0 'no' */
+  int32_t nip;         /* OPTS_D.nip: 1, 4, 5, 8, 10, 11 or 14 [4] */
+  double tol;          /* OPTS_D.tol [1e-8] */
+  int32_t is_abs_tol;  /* OPTS_D.is_abs_tol: 0 relative to the initial residual [0] */
+  int32_t itmax;       /* OPTS_D.itmax [500] */
+} m3_diffusion_opts;
+
+typedef struct {
+  int32_t nit;         /* PCG iterations (profiling.nit of pcg_solver_p) */
+  double norm_r0;      /* initial residual (RMS over the free nodes) */
+  double norm_r;       /* final residual */
+  double tol;          /* absolute tolerance used */
+  double t_assemble, t_solve; /* s */
+} m3_diffusion_info;
+
+/* Linear tetrahedra of the diffusion step: EL2NOD is 4 x nel, 1-based, over
+ * the first nnodT nodes of the finest mesh (the P1 vertices with the vertex
+ * tetrahedra, or all P2 nodes with each 10-node element split into 8 linear
+ * ones as diffusion3d_p.m:67-70 does for OPTS_D.eltype = 'linear').  Element
+ * classes are computed on these tetrahedra (check_els_in_cone.m,
+ * check_phi.m; assembly_diffusion_3d_double_jacobian.m:73-93) with the cone
+ * half-aperture theta_cone in degrees (SETTINGS.theta_cone). */
+int m3_diffusion_mesh(m3_ctx *ctx, const uint32_t *EL2NOD, int64_t nel, int64_t nnodT, double theta_cone);
+
+/* T = diffusion3d_p(T,kappa,MESH,COMM,SETTINGS,TBC,dt) (diffusion3d_p.m:1-116):
+ * conductivity and mass matrices of the double-Jacobian linear elements
+ * (assembly_diffusion_3d_double_jacobian.m, class kernels
+ * mfiles_SPH/assembly_diffusion_els_*_opt.m), lumping, the theta rule, the
+ * fixed temperatures TBC.ifix / TBC.vfix (1-based node indices) and the
+ * Jacobi-preconditioned CG of pcg_solver_p.m:49-186.  T (nnodT) is updated
+ * in place.  kappa = 0 returns T unchanged, as the reference does. */
+int m3_diffusion3d(m3_ctx *ctx, double *T, double kappa, double dt, const int64_t *ifix, const double *vfix,
+                   int64_t nfix, const m3_diffusion_opts *opts, m3_diffusion_info *info);
+
+/* Operators of the last m3_diffusion3d call for parity checks: y = CC x
+ * (which 0), y = MM x (which 1, the consistent mass matrix), y = MM_lumped x
+ * (which 2).  x, y: nnodT.  No reference counterpart (diagnostics). */
+int m3_diffusion_apply(m3_ctx *ctx, int32_t which, const double *x, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,35 @@ def rock_params(Visc0=1.0, Dens0=1.0, T_ref=0.0, alpha=0.0, Ea=0.0, Va=0.0, g=-1
     return p
 
 
+class DiffusionOpts(C.Structure):
+    """SETTINGS.OPTS_D (default_numerics_M3TET.m:95-134), m3_diffusion_opts."""
+    _fields_ = [("theta", C.c_double), ("lumping", C.c_int32), ("nip", C.c_int32), ("tol", C.c_double),
+                ("is_abs_tol", C.c_int32), ("itmax", C.c_int32)]
+
+
+class DiffusionInfo(C.Structure):
+    _fields_ = [("nit", C.c_int32), ("norm_r0", C.c_double), ("norm_r", C.c_double), ("tol", C.c_double),
+                ("t_assemble", C.c_double), ("t_solve", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def diffusion_opts(theta=1.0, lumping="yes", nip=4, tol=1e-8, is_abs_tol=0, itmax=500):
+    """Defaults of default_numerics_M3TET.m:95-134 (eltype 'linear', pc 'diag')."""
+    o = DiffusionOpts()
+    o.theta, o.nip, o.tol, o.is_abs_tol, o.itmax = theta, nip, tol, int(is_abs_tol), itmax
+    o.lumping = int(lumping in ("yes", True, 1))
+    return o
+
+
+LIB.m3_diffusion_mesh.argtypes = [C.c_void_p, U32, C.c_int64, C.c_int64, C.c_double]
+LIB.m3_diffusion3d.argtypes = [C.c_void_p, D, C.c_double, C.c_double, I64, D, C.c_int64,
+                               C.POINTER(DiffusionOpts), C.POINTER(DiffusionInfo)]
+LIB.m3_diffusion_apply.argtypes = [C.c_void_p, C.c_int32, D, D]
+for _f in ("m3_diffusion_mesh", "m3_diffusion3d", "m3_diffusion_apply"):
+    getattr(LIB, _f).restype = C.c_int
+
 LIB.m3_rock_prop.argtypes = [C.c_void_p, D, C.POINTER(RockParams), D, D]
 LIB.m3_calculate_dt.argtypes = [C.c_void_p, D, C.c_double, C.c_double, D]
 LIB.m3_rock_prop.restype = C.c_int
@@ -339,6 +368,36 @@ class StokesContext:
         self._check(LIB.m3_sl_advect(self._h, _p(T), _p(U), float(dt), float(T_cmb), float(T_surf), _p(out)))
         return out
 
+    def diffusion_mesh(self, EL4=None, nnodT=None, theta_cone=None):
+        """Linear tetrahedra of the diffusion step (0-based (nel, 4) over the
+        first nnodT nodes).  Default: the finest mesh's vertex tetrahedra on
+        the P1 vertices, the grid the temperature lives on here."""
+        m = self.mesh
+        if EL4 is None:
+            EL4, nnodT = m.levels[0].EL2NOD[:, :4], m.nVnod
+        EL = np.ascontiguousarray(np.asarray(EL4, dtype=np.int64) + 1, dtype=np.uint32)
+        tc = m.theta_cone if theta_cone is None else theta_cone
+        self._check(LIB.m3_diffusion_mesh(self._h, _p(EL, U32), len(EL), int(nnodT), float(tc)))
+        self._diff_n = int(nnodT)
+
+    def diffusion3d(self, T, kappa, dt, ifix=(), vfix=(), **opts):
+        """T = diffusion3d_p(T,kappa,...,TBC,dt) (diffusion3d_p.m:1-116) on the
+        tetrahedra of ``diffusion_mesh``; ifix 0-based, vfix the values."""
+        T = np.ascontiguousarray(T, dtype=np.float64).copy()
+        ifx = np.ascontiguousarray(np.asarray(ifix, dtype=np.int64) + 1)
+        vfx = np.ascontiguousarray(np.broadcast_to(np.asarray(vfix, dtype=np.float64), ifx.shape))
+        o = diffusion_opts(**opts)
+        info = DiffusionInfo()
+        self._check(LIB.m3_diffusion3d(self._h, _p(T), float(kappa), float(dt), _p(ifx, I64), _p(vfx),
+                                       len(ifx), C.byref(o), C.byref(info)))
+        return T, info.as_dict()
+
+    def diffusion_apply(self, which, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._check(LIB.m3_diffusion_apply(self._h, int(which), _p(x), _p(y)))
+        return y
+
     def apply(self, which, level, x):
         m = self.mesh
         sizes = {0: 3 * m.levels[level].nnod, 1: 3 * m.nnod, 2: m.nVnod, 3: m.nVnod}
@@ -484,3 +543,33 @@ def get_operator(ctx, level):
     val = np.zeros(9 * nb)
     ctx._check(LIB.m3_get_operator(ctx._h, level, _p(rp, I64), _p(col, I32), _p(val)))
     return sp.bsr_matrix((val.reshape(nb, 3, 3), col, rp), shape=(3 * nn, 3 * nn))
+
+
+def diffusion3d_p(T, kappa, MESH, COMM, SETTINGS, TBC, dt):
+    """[T] = diffusion3d_p(T,kappa,MESH,COMM,SETTINGS,TBC,dt) (diffusion3d_p.m:1)
+    on the vertex tetrahedra of MESH.  SETTINGS["OPTS_D"] holds theta,
+    lumping, nip, tol, is_abs_tol, itmax; TBC["ifix"] (0-based) and
+    TBC["vfix"] the fixed temperatures."""
+    if np.size(np.unique(kappa)) > 1:
+        raise M3Error(1, "Diffusivity kappa must be a scalar.")
+    ctx = _ctx(MESH)
+    if getattr(ctx, "_diff_n", None) != len(T):
+        ctx.diffusion_mesh()
+    o = {k: v for k, v in (SETTINGS.get("OPTS_D") or {}).items()
+         if k in ("theta", "lumping", "nip", "tol", "is_abs_tol", "itmax")}
+    T, _ = ctx.diffusion3d(T, float(np.ravel(kappa)[0]), dt, TBC.get("ifix", ()), TBC.get("vfix", ()), **o)
+    return T
+
+
+def thermal_advdiff_3d_p(VAR, MESH, COMM, PHYSICS, SETTINGS, NUMSCALE, TBC, dt):
+    """[VAR] = thermal_advdiff_3d_p(...) (thermal_advdiff_3d_p.m:1-56, 'Diffusion'
+    solver, curved edges): diffusion over dt/2, semi-Lagrangian advection over
+    dt with the rotated-frame velocity VAR["U"], diffusion over dt/2.  VAR["T"]
+    holds the vertex temperatures."""
+    ctx = _ctx(MESH)
+    T = diffusion3d_p(VAR["T"], PHYSICS["kappa"], MESH, COMM, SETTINGS, TBC, dt / 2)
+    T = ctx.sl_advect(T, VAR["U"], dt, SETTINGS.get("T_cmb", 1.0), SETTINGS.get("T_surf", 0.0))
+    T = diffusion3d_p(T, PHYSICS["kappa"], MESH, COMM, SETTINGS, TBC, dt / 2)
+    VAR = dict(VAR)
+    VAR["T"] = T
+    return VAR

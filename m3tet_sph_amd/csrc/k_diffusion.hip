@@ -1,0 +1,495 @@
+// Thermal diffusion step of the time loop on linear tetrahedra, the
+// 'Diffusion' branch of thermal_advdiff_3d_p.m:13-20,46-52:
+// diffusion3d_p.m:1-116 with the double-Jacobian element matrices of
+// assembly_diffusion_3d_double_jacobian.m and its four class kernels
+// (mfiles_SPH/assembly_diffusion_els_*_opt.m) and the Jacobi-preconditioned
+// CG of pcg_solver_p.m.
+//
+// One thread per element builds the 4x4 conductivity and mass matrices
+//   CC = sum_ip kappa w dN/dx . dN/dx,   MM = sum_ip w N N',
+//   w = detJ_CS detJ_SL w_ip,  dN/dx = J_SC inv(J_SL) dN/dxi
+// in the spherical frame of its class (GCOORD_SPH; phi + pi for elements
+// crossing phi = 2pi; GCOORD rotated 90 degrees about X for elements touching
+// the cone).  CC and MM do not depend on the frame, so nothing is rotated
+// back.  The values are added into a scalar sliced (SELL-64) pattern through
+// a per-element slot map, and the lumped mass (row sums) into a vector.  The
+// system (MM + dt theta CC) T = MM T0 - dt (1 - theta) CC T0 is then solved
+// on the free nodes: the fixed rows and columns are replaced by the identity
+// and their known values are moved to the right-hand side.
+#include "m3_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace {
+
+constexpr int MAXIP = 14;
+
+struct IPTab {
+  int nip;
+  double r[MAXIP], s[MAXIP], t[MAXIP], w[MAXIP];
+};
+
+// ip_tetrahedron.m:23-122, weights / 6 (:189-191)
+bool ip_rule(int nip, IPTab &T) {
+  T.nip = nip;
+  auto set = [&](int n, const double *r, const double *s, const double *t, const double *w) {
+    for (int i = 0; i < n; ++i) {
+      T.r[i] = r[i];
+      T.s[i] = s[i];
+      T.t[i] = t[i];
+      T.w[i] = w[i] / 6.0;
+    }
+  };
+  switch (nip) {
+    case 1: {
+      const double r[1] = {0.25}, w[1] = {1.0};
+      set(1, r, r, r, w);
+      return true;
+    }
+    case 4: {
+      const double a = (5 + 3 * std::sqrt(5.0)) / 20, b = (5 - std::sqrt(5.0)) / 20;
+      const double r[4] = {a, b, b, b}, s[4] = {b, a, b, b}, t[4] = {b, b, a, b}, w[4] = {0.25, 0.25, 0.25, 0.25};
+      set(4, r, s, t, w);
+      return true;
+    }
+    case 5: {
+      const double r[5] = {1.0 / 4, 1.0 / 2, 1.0 / 6, 1.0 / 6, 1.0 / 6};
+      const double s[5] = {1.0 / 4, 1.0 / 6, 1.0 / 2, 1.0 / 6, 1.0 / 6};
+      const double t[5] = {1.0 / 4, 1.0 / 6, 1.0 / 6, 1.0 / 2, 1.0 / 6};
+      const double w[5] = {-4.0 / 5, 9.0 / 20, 9.0 / 20, 9.0 / 20, 9.0 / 20};
+      set(5, r, s, t, w);
+      return true;
+    }
+    case 8: {
+      const double q = 1.0 / 3;
+      const double r[8] = {0, q, q, q, 1, 0, 0, 0}, s[8] = {q, 0, q, q, 0, 1, 0, 0};
+      const double t[8] = {q, q, 0, q, 0, 0, 1, 0};
+      const double w[8] = {9.0 / 40, 9.0 / 40, 9.0 / 40, 9.0 / 40, 1.0 / 40, 1.0 / 40, 1.0 / 40, 1.0 / 40};
+      set(8, r, s, t, w);
+      return true;
+    }
+    case 10: {
+      const double a = 0.7784952948213300, b = 0.0738349017262234, c = 0.4062443438840510,
+                   d = 0.0937556561159491, w1 = 0.0476331348432089, w2 = 0.1349112434378610;
+      const double r[10] = {a, b, b, b, c, c, c, d, d, d}, s[10] = {b, a, b, b, c, d, d, c, c, d};
+      const double t[10] = {b, b, a, b, d, c, d, c, d, c};
+      const double w[10] = {w1, w1, w1, w1, w2, w2, w2, w2, w2, w2};
+      set(10, r, s, t, w);
+      return true;
+    }
+    case 11: {
+      const double a = (1 + std::sqrt(5.0 / 14)) / 4, b = (1 - std::sqrt(5.0 / 14)) / 4;
+      const double w1 = -148.0 / 1875, w2 = 343.0 / 7500, w3 = 56.0 / 375;
+      const double e = 11.0 / 14, f = 1.0 / 14;
+      const double r[11] = {0.25, e, f, f, f, a, a, a, b, b, b}, s[11] = {0.25, f, e, f, f, a, b, b, a, a, b};
+      const double t[11] = {0.25, f, f, e, f, b, a, b, a, b, a};
+      const double w[11] = {w1, w2, w2, w2, w2, w3, w3, w3, w3, w3, w3};
+      set(11, r, s, t, w);
+      return true;
+    }
+    case 14: {
+      const double g = 1.0 / (46.0 * std::sqrt(46.0));
+      const double h = std::acos(g) + (2.0 / 3.0) * std::asin(g);
+      const double k = (104.0 + 8.0 * std::sqrt(46.0) * std::cos(h)) / 3.0;
+      const double sq = std::sqrt(49.0 - k);
+      const double b = (7.0 + sq) / k, a = 1.0 - 3.0 * b, d = (7.0 - sq) / k, c = 1.0 - 3.0 * d;
+      const double p = (98.0 - k - 14.0 * sq) / (1680.0 * sq * std::pow(b - a, 3));
+      const double q = (98.0 - k + 14.0 * sq) / (1680.0 * sq * std::pow(c - d, 3));
+      const double rr = (1.0 - 4.0 * (p + q)) / 6.0;
+      const double e = (1.0 + std::pow(2.0 / (105.0 * rr), 0.25)) / 4.0, f = (1.0 - 2.0 * e) / 2.0;
+      const double r[14] = {a, b, b, b, c, d, d, d, e, e, e, f, f, f};
+      const double s[14] = {b, a, b, b, d, c, d, d, e, f, f, e, e, f};
+      const double t[14] = {b, b, a, b, d, d, c, d, f, e, f, e, f, e};
+      const double w[14] = {p, p, p, p, q, q, q, q, rr, rr, rr, rr, rr, rr};
+      set(14, r, s, t, w);
+      return true;
+    }
+    default:
+      return false;
+  }
+}
+
+struct DiffArgs {
+  int64_t nel;
+  const int32_t *el;
+  const uint8_t *cls;
+  const double *X, *S;
+  const int32_t *slots;
+  double kappa;
+  IPTab ip;
+  double *C, *M, *ML;
+  int *err;
+};
+
+__device__ inline void cart2sph_d(double x, double y, double z, double &th, double &ph, double &r) {
+  r = sqrt(x * x + y * y + z * z);
+  th = atan2(sqrt(x * x + y * y), z);
+  ph = atan2(y, x);
+  if (ph < 0) ph += 2.0 * M_PI;
+}
+
+__global__ __launch_bounds__(256) void k_diff_elem(DiffArgs a) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= a.nel) return;
+  int n[4];
+  for (int i = 0; i < 4; ++i) n[i] = a.el[4 * e + i];
+  const int cls = a.cls[e];
+  double th[4], ph[4], rr[4];
+  for (int i = 0; i < 4; ++i) {
+    if (cls >= 2) {  // 90-degree rotation about X: x' = x, y' = -z, z' = y
+      const double *x = a.X + 3 * (int64_t)n[i];
+      cart2sph_d(x[0], -x[2], x[1], th[i], ph[i], rr[i]);
+    } else {
+      const double *s = a.S + 3 * (int64_t)n[i];
+      th[i] = s[0];
+      ph[i] = s[1];
+      rr[i] = s[2];
+      if (cls == 1) {  // 180-degree rotation about Z
+        ph[i] += M_PI;
+        if (ph[i] > 2.0 * M_PI) ph[i] -= 2.0 * M_PI;
+      }
+    }
+  }
+  // J[k][m] = sum_n dN_n/dxi_k X_m(n) = X_m(k) - X_m(4th vertex)
+  double J[3][3];
+  for (int k = 0; k < 3; ++k) {
+    J[k][0] = th[k] - th[3];
+    J[k][1] = ph[k] - ph[3];
+    J[k][2] = rr[k] - rr[3];
+  }
+  const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  if (det < 0) atomicOr(a.err, 1);
+  const double id = 1.0 / det;
+  double I[3][3];  // I[m][k] = d xi_k / d sph_m
+  I[0][0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+  I[0][1] = id * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+  I[0][2] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+  I[1][0] = id * (J[1][2] * J[2][0] - J[1][0] * J[2][2]);
+  I[1][1] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+  I[1][2] = id * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+  I[2][0] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  I[2][1] = id * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+  I[2][2] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+  double C[4][4] = {}, M[4][4] = {};
+  for (int ip = 0; ip < a.ip.nip; ++ip) {
+    const double N[4] = {a.ip.r[ip], a.ip.s[ip], a.ip.t[ip], 1.0 - a.ip.r[ip] - a.ip.s[ip] - a.ip.t[ip]};
+    const double thi = th[0] * N[0] + th[1] * N[1] + th[2] * N[2] + th[3] * N[3];
+    const double phi = ph[0] * N[0] + ph[1] * N[1] + ph[2] * N[2] + ph[3] * N[3];
+    const double ri = rr[0] * N[0] + rr[1] * N[1] + rr[2] * N[2] + rr[3] * N[3];
+    if (thi == 0.0) atomicOr(a.err, 2);
+    const double st = sin(thi), ct = cos(thi), sp = sin(phi), cp = cos(phi);
+    const double dCS = ri * ri * st;
+    const double idCS = 1.0 / dCS;
+    // J_SC[i][m] = d sph_m / d x_i
+    const double SC[3][3] = {{ri * st * ct * cp * idCS, -ri * sp * idCS, ri * ri * st * st * cp * idCS},
+                             {ri * st * ct * sp * idCS, ri * cp * idCS, ri * ri * st * st * sp * idCS},
+                             {-ri * st * st * idCS, 0.0, ri * ri * st * ct * idCS}};
+    double D[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int k = 0; k < 3; ++k) D[i][k] = SC[i][0] * I[0][k] + SC[i][1] * I[1][k] + SC[i][2] * I[2][k];
+    double g[4][3];
+    for (int i = 0; i < 3; ++i) {
+      g[0][i] = D[i][0];
+      g[1][i] = D[i][1];
+      g[2][i] = D[i][2];
+      g[3][i] = -(D[i][0] + D[i][1] + D[i][2]);
+    }
+    const double w = dCS * det * a.ip.w[ip];
+    for (int p = 0; p < 4; ++p)
+      for (int q = p; q < 4; ++q) {
+        C[p][q] += a.kappa * w * (g[p][0] * g[q][0] + g[p][1] * g[q][1] + g[p][2] * g[q][2]);
+        M[p][q] += w * N[p] * N[q];
+      }
+  }
+  const int32_t *slot = a.slots + 16 * e;
+  for (int p = 0; p < 4; ++p) {
+    double ml = 0;
+    for (int q = 0; q < 4; ++q) {
+      const int lo = p < q ? p : q, hi = p < q ? q : p;
+      atomicAdd(a.C + slot[4 * p + q], C[lo][hi]);
+      atomicAdd(a.M + slot[4 * p + q], M[lo][hi]);
+      ml += M[lo][hi];
+    }
+    atomicAdd(a.ML + n[p], ml);
+  }
+}
+
+struct SysArgs {
+  SellView C;
+  const double *M, *ML;
+  int lumped;
+  int64_t n;
+  double dt, theta;
+  const double *mask, *vfix, *T;
+};
+
+// mass coefficient of stored slot p (row i, column j)
+__device__ inline double mass_at(const SysArgs &a, int64_t p, int64_t i, int64_t j) {
+  return a.lumped ? (i == j ? a.ML[i] : 0.0) : a.M[p];
+}
+
+// rhs = MM T - dt (1 - theta) CC T - A(:, fixed) vfix   (diffusion3d_p.m:91-100)
+__global__ void k_diff_rhs(SysArgs a, double *rhs) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t base = a.C.slice_ptr[i >> 6] + (i & 63);
+  const int len = a.C.rowlen[i];
+  double sM = 0, sC = 0, lift = 0;
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = base + (int64_t)k * 64;
+    const int64_t j = a.C.col[p];
+    const double m = mass_at(a, p, i, j), cc = a.C.val[p];
+    sM += m * a.T[j];
+    sC += cc * a.T[j];
+    if (a.mask[j] == 0.0) lift += (a.dt * a.theta * cc + m) * a.vfix[j];
+  }
+  rhs[i] = sM - a.dt * (1.0 - a.theta) * sC - lift;
+}
+
+// A = dt theta CC + MM with identity rows/columns at the fixed nodes,
+// Adi = 1 / diag(A); x = T at the free nodes, 0 at the fixed ones
+__global__ void k_diff_sys(SysArgs a, double *Aval, double *Adi, double *x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t base = a.C.slice_ptr[i >> 6] + (i & 63);
+  const int len = a.C.rowlen[i];
+  const bool fi = a.mask[i] != 0.0;
+  double diag = 1.0;
+  for (int k = 0; k < len; ++k) {
+    const int64_t p = base + (int64_t)k * 64;
+    const int64_t j = a.C.col[p];
+    double v = a.dt * a.theta * a.C.val[p] + mass_at(a, p, i, j);
+    if (!fi || a.mask[j] == 0.0) v = (i == j) ? 1.0 : 0.0;
+    Aval[p] = v;
+    if (i == j) diag = v;
+  }
+  Adi[i] = 1.0 / diag;
+  x[i] = fi ? a.T[i] : 0.0;
+}
+
+// r = mask (rhs - Ax)
+__global__ void k_diff_res(int64_t n, const double *mask, const double *rhs, const double *Ax, double *r) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) r[i] = mask[i] * (rhs[i] - Ax[i]);
+}
+
+// T = x at the free nodes, vfix at the fixed ones (diffusion3d_p.m:102,116)
+__global__ void k_diff_out(int64_t n, const double *mask, const double *x, const double *vfix, double *T) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) T[i] = mask[i] != 0.0 ? x[i] : vfix[i];
+}
+
+__global__ void k_diff_mul_diag(int64_t n, const double *d, const double *x, double *y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = d[i] * x[i];
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+int32_t find_slot(const Sell &S, int64_t row, int64_t col) {
+  const int64_t base = S.h_slice_ptr[row >> 6] + (row & 63);
+  int lo = 0, hi = S.h_rowlen[row] - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int32_t cv = S.h_col[base + (int64_t)mid * 64];
+    if (cv == col) return (int32_t)(base + (int64_t)mid * 64);
+    if (cv < col) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+using clk = std::chrono::steady_clock;
+
+}  // namespace
+
+namespace m3 {
+
+void diffusion_mesh(m3_ctx *c, const uint32_t *EL2NOD, int64_t nel, int64_t nnodT, double theta_cone) {
+  M3_REQUIRE(!c->L.empty(), M3_E_STATE, "m3_diffusion_mesh before m3_set_mesh");
+  M3_REQUIRE(EL2NOD && nel > 0, M3_E_ARG, "empty element list");
+  M3_REQUIRE(nnodT > 0 && nnodT <= c->nnod, M3_E_ARG, "nnodT must be in 1..nnod of the finest mesh");
+  M3_REQUIRE(theta_cone > 0 && theta_cone < 90, M3_E_ARG, "theta_cone must be in (0, 90) degrees");
+  auto &D = c->dif;
+  D.ready = D.assembled = false;
+  std::vector<int32_t> el(4 * nel);
+  for (int64_t k = 0; k < 4 * nel; ++k) {
+    const int64_t v = (int64_t)EL2NOD[k] - 1;
+    M3_REQUIRE(v >= 0 && v < nnodT, M3_E_MESH, "EL2NOD entry out of range");
+    el[k] = (int32_t)v;
+  }
+  // classes (check_els_in_cone.m, check_phi.m on the 4 vertices)
+  const double tc = theta_cone * M_PI / 180.0;
+  std::vector<uint8_t> cls(nel);
+  for (int64_t e = 0; e < nel; ++e) {
+    int cnt = 0;
+    double ph[4];
+    for (int i = 0; i < 4; ++i) {
+      const double th = c->h_sph[3 * (int64_t)el[4 * e + i]];
+      cnt += (th <= tc || th >= M_PI - tc) ? 1 : 0;
+      ph[i] = c->h_sph[3 * (int64_t)el[4 * e + i] + 1];
+    }
+    bool cross = false;
+    const int pr[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+    for (auto &q : pr) cross = cross || std::fabs(ph[q[0]] - ph[q[1]]) > M_PI;
+    cls[e] = cnt == 0 ? (cross ? 1 : 0) : (cnt >= 3 ? 2 : 3);
+  }
+  // node adjacency through the elements
+  std::vector<std::vector<int32_t>> rows(nnodT);
+  for (int64_t e = 0; e < nel; ++e)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) rows[el[4 * e + i]].push_back(el[4 * e + j]);
+  std::vector<int64_t> ptr(nnodT + 1, 0);
+  std::vector<int32_t> cols;
+  for (int64_t r = 0; r < nnodT; ++r) {
+    auto &v = rows[r];
+    M3_REQUIRE(!v.empty(), M3_E_MESH, "a node of 1..nnodT belongs to no element");
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    cols.insert(cols.end(), v.begin(), v.end());
+    ptr[r + 1] = (int64_t)cols.size();
+    std::vector<int32_t>().swap(v);
+  }
+  D.C.build(nnodT, nnodT, ptr, cols, 1);
+  std::vector<int32_t> slots(16 * nel);
+  for (int64_t e = 0; e < nel; ++e)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) {
+        const int32_t sl = find_slot(D.C, el[4 * e + i], el[4 * e + j]);
+        M3_REQUIRE(sl >= 0, M3_E_MESH, "element pair missing from the diffusion pattern");
+        slots[16 * e + 4 * i + j] = sl;
+      }
+  D.el.upload(el);
+  D.cls.upload(cls);
+  D.slots.upload(slots);
+  const size_t ns = (size_t)D.C.nslot, n = (size_t)nnodT;
+  D.M.alloc(ns);
+  D.A.alloc(ns);
+  for (auto *b : {&D.ML, &D.Adi, &D.mask, &D.vfix, &D.T, &D.rhs, &D.x, &D.r, &D.rl, &D.d, &D.q, &D.Aq}) b->alloc(n);
+  // padding slots of the value arrays are never read; clear them once anyway
+  D.C.val.zero(c->stream);
+  D.M.zero(c->stream);
+  D.A.zero(c->stream);
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  D.nel = nel;
+  D.n = nnodT;
+  D.theta_cone = theta_cone;
+  D.ready = true;
+}
+
+void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *ifix, const double *vfix,
+                 int64_t nfix, const m3_diffusion_opts *o, m3_diffusion_info *info) {
+  auto &D = c->dif;
+  M3_REQUIRE(D.ready, M3_E_STATE, "m3_diffusion3d before m3_diffusion_mesh");
+  M3_REQUIRE(T && o, M3_E_ARG, "null argument");
+  M3_REQUIRE(nfix >= 0 && (nfix == 0 || (ifix && vfix)), M3_E_ARG, "fixed-node arrays missing");
+  M3_REQUIRE(o->theta >= 0.0 && o->theta <= 1.0, M3_E_ARG, "OPTS_D.theta must be in [0, 1]");
+  M3_REQUIRE(o->itmax >= 1, M3_E_ARG, "OPTS_D.itmax must be >= 1");
+  M3_REQUIRE(std::isfinite(kappa) && std::isfinite(dt) && dt >= 0, M3_E_ARG, "kappa and dt must be finite, dt >= 0");
+  IPTab ipt{};
+  M3_REQUIRE(ip_rule(o->nip, ipt), M3_E_ARG, "Unkown integration rule.");
+  if (info) *info = m3_diffusion_info{};
+  if (kappa == 0.0) return;  // diffusion3d_p.m:45-52
+  hipStream_t s = c->stream;
+  const int64_t n = D.n;
+  auto t0 = clk::now();
+  // element matrices
+  D.C.val.zero(s);
+  D.M.zero(s);
+  D.ML.zero(s);
+  c->errflag.zero(s);
+  DiffArgs a{};
+  a.nel = D.nel;
+  a.el = D.el.p;
+  a.cls = D.cls.p;
+  a.X = c->X.p;
+  a.S = c->S.p;
+  a.slots = D.slots.p;
+  a.kappa = kappa;
+  a.ip = ipt;
+  a.C = D.C.val.p;
+  a.M = D.M.p;
+  a.ML = D.ML.p;
+  a.err = c->errflag.p;
+  hipLaunchKernelGGL(k_diff_elem, dim3(nblk(D.nel)), dim3(256), 0, s, a);
+  HIPCHECK(hipGetLastError());
+  int err = 0;
+  HIPCHECK(hipMemcpyAsync(&err, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (err & 1) throw M3Error(M3_E_NUMERIC, "negative Jacobi");
+  if (err & 2) throw M3Error(M3_E_NUMERIC, "integration point has theta = 0 (det(J_CS) = Inf)");
+  D.assembled = true;
+  // fixed nodes
+  std::vector<double> mask(n, 1.0), vf(n, 0.0);
+  for (int64_t k = 0; k < nfix; ++k) {
+    const int64_t i = ifix[k] - 1;
+    M3_REQUIRE(i >= 0 && i < n, M3_E_ARG, "TBC.ifix out of range");
+    mask[i] = 0.0;
+    vf[i] = vfix[k];
+  }
+  int64_t nfree = 0;
+  for (double m : mask) nfree += m != 0.0;
+  D.mask.upload(mask);
+  D.vfix.upload(vf);
+  D.T.upload(T, n);
+  const double t_asm = std::chrono::duration<double>(clk::now() - t0).count();
+  auto t1 = clk::now();
+  SysArgs sa{D.C.view(), D.M.p, D.ML.p, o->lumping ? 1 : 0, n, dt, o->theta, D.mask.p, D.vfix.p, D.T.p};
+  hipLaunchKernelGGL(k_diff_rhs, dim3(nblk(n)), dim3(256), 0, s, sa, D.rhs.p);
+  hipLaunchKernelGGL(k_diff_sys, dim3(nblk(n)), dim3(256), 0, s, sa, D.A.p, D.Adi.p, D.x.p);
+  SellView Av = D.C.view();
+  Av.val = D.A.p;
+  m3::spmv_scalar(s, Av, n, D.x.p, D.Aq.p);
+  hipLaunchKernelGGL(k_diff_res, dim3(nblk(n)), dim3(256), 0, s, n, D.mask.p, D.rhs.p, D.Aq.p, D.r.p);
+  HIPCHECK(hipGetLastError());
+  int nit = 0;
+  double norm_r = 0, norm_r0 = 0, tol = o->tol;
+  if (nfree > 0) {
+    norm_r0 = std::sqrt(m3::dot(c, D.r.p, D.r.p, n) / (double)nfree);  // normdf (normdf.m)
+    if (std::isnan(norm_r0)) throw M3Error(M3_E_NUMERIC, "NaN in the diffusion right-hand side");
+    tol = o->is_abs_tol ? o->tol : o->tol * norm_r0;
+    norm_r = norm_r0;
+    // a zero initial residual is already the solution (the reference would
+    // divide 0 / 0 in its first step)
+    if (norm_r0 > 0.0 && !(norm_r0 < tol))
+      nit = m3::pcg_diag_run(c, Av, n, D.Adi.p, D.x.p, D.r.p, D.rl.p, D.d.p, D.q.p, D.Aq.p, (double)nfree, tol,
+                             o->itmax, &norm_r);
+  }
+  hipLaunchKernelGGL(k_diff_out, dim3(nblk(n)), dim3(256), 0, s, n, D.mask.p, D.x.p, D.vfix.p, D.T.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  D.T.download(T, n);
+  if (info) {
+    info->nit = nit;
+    info->norm_r0 = norm_r0;
+    info->norm_r = norm_r;
+    info->tol = tol;
+    info->t_assemble = t_asm;
+    info->t_solve = std::chrono::duration<double>(clk::now() - t1).count();
+  }
+}
+
+void diffusion_apply(m3_ctx *c, int which, const double *x, double *y) {
+  auto &D = c->dif;
+  M3_REQUIRE(D.assembled, M3_E_STATE, "no diffusion operators yet (call m3_diffusion3d)");
+  M3_REQUIRE(x && y, M3_E_ARG, "null argument");
+  M3_REQUIRE(which >= 0 && which <= 2, M3_E_ARG, "which must be 0, 1 or 2");
+  hipStream_t s = c->stream;
+  const int64_t n = D.n;
+  D.d.upload(x, n);
+  SellView v = D.C.view();
+  if (which == 1) v.val = D.M.p;
+  if (which == 2)
+    hipLaunchKernelGGL(k_diff_mul_diag, dim3(nblk(n)), dim3(256), 0, s, n, D.ML.p, D.d.p, D.q.p);
+  else
+    m3::spmv_scalar(s, v, n, D.d.p, D.q.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(s));
+  D.q.download(y, n);
+}
+
+}  // namespace m3

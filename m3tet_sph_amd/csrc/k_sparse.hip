@@ -444,6 +444,44 @@ __global__ void k_add_scalar(int64_t n, double *a, double v) {
   if (i < n) a[i] += v;
 }
 
+// scalar SELL SpMV y = A x (one thread per row) with per-block partial sums
+// of x.y when part != nullptr
+__global__ __launch_bounds__(RB) void k_spmv1(SellView A, int64_t nrow, const double *x, double *y,
+                                              double *part) {
+  const int64_t row = blockIdx.x * (int64_t)RB + threadIdx.x;
+  double a = 0;
+  if (row < nrow) {
+    const int64_t base = A.slice_ptr[row >> 6] + (row & 63);
+    const int len = A.rowlen[row];
+    for (int k = 0; k < len; ++k) {
+      const int64_t p = base + (int64_t)k * 64;
+      a += A.val[p] * x[A.col[p]];
+    }
+    y[row] = a;
+  }
+  if (!part) return;
+  double s = row < nrow ? x[row] * a : 0.0;
+  __shared__ double sh[RB];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = RB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Jacobi preconditioner d = Adi .* r
+__global__ void k_precond_diag(int64_t n, const double *r, const double *Adi, double *d) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) d[i] = r[i] * Adi[i];
+}
+
+__global__ void k_zero64(int64_t n, double *x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.0;
+}
+
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -756,6 +794,7 @@ int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, in
   if (hist) hist->push_back(rrms);
   int it = 0;
   const unsigned gK = nblk(f.nnod);
+  hipLaunchKernelGGL(k_zero64, dim3(nblk(n3)), dim3(256), 0, s, n3, q);  // q = d + 0 q on the first step
   for (it = 1; it <= itmax; ++it) {
     vcycle(c, r, d, precision);
     hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(RB), 0, s, n3, r, it == 1 ? (const double *)nullptr : rl, d,
@@ -774,6 +813,45 @@ int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, in
     if (hist) hist->push_back(rrms);
     if (std::isnan(rrms)) throw M3Error(M3_E_NUMERIC, "NaN in CG residual");
     if (rrms < tol) break;
+  }
+  HIPCHECK(hipGetLastError());
+  return it > itmax ? itmax : it;
+}
+
+void spmv_scalar(hipStream_t s, SellView A, int64_t n, const double *x, double *y) {
+  hipLaunchKernelGGL(k_spmv1, dim3(nblk(n, RB)), dim3(RB), 0, s, A, n, x, y, (double *)nullptr);
+  HIPCHECK(hipGetLastError());
+}
+
+// pcg_solver_p.m:49-186 with the 'diag' preconditioner.  The device-scalar
+// program of mgpcg_run (Polak-Ribiere beta, alpha = r.d / q.Aq) on scalars
+// 16.. of the context's scalar block; one host read of |r|^2 per iteration.
+int pcg_diag_run(m3_ctx *c, SellView A, int64_t n, const double *Adi, double *x, double *r, double *rl,
+                 double *d, double *q, double *Aq, double nfree, double tol, int itmax, double *norm) {
+  hipStream_t s = c->stream;
+  double *sc = c->scal.p + 16;
+  const int nb = red_blocks(n);
+  const unsigned gA = nblk(n, RB);
+  hipLaunchKernelGGL(k_zero64, dim3(nblk(n)), dim3(256), 0, s, n, q);
+  int it = 0;
+  double rr = 0;
+  for (it = 1; it <= itmax; ++it) {
+    hipLaunchKernelGGL(k_precond_diag, dim3(nblk(n)), dim3(256), 0, s, n, r, Adi, d);
+    hipLaunchKernelGGL(k_dot2, dim3(nb), dim3(RB), 0, s, n, r, it == 1 ? (const double *)nullptr : rl, d,
+                       c->partial.p, nb);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(RB), 0, s, c->partial.p, nb, sc, it == 1 ? 1 : 0);
+    hipLaunchKernelGGL(k_cg_dir, dim3(nblk(n)), dim3(256), 0, s, n, d, q, sc);
+    hipLaunchKernelGGL(k_spmv1, dim3(gA), dim3(RB), 0, s, A, n, q, Aq, c->partial.p);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(RB), 0, s, c->partial.p, (int)gA, sc);
+    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(RB), 0, s, n, x, q, r, rl, Aq, sc, c->partial.p);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, sc + 5);
+    HIPCHECK(hipMemcpyAsync(c->h_scal + 21, sc + 5, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    rr = c->h_scal[21];
+    const double nr = std::sqrt(rr / nfree);
+    if (std::isnan(nr)) throw M3Error(M3_E_NUMERIC, "NaN in CG residual");
+    *norm = nr;
+    if (nr < tol) break;
   }
   HIPCHECK(hipGetLastError());
   return it > itmax ? itmax : it;

@@ -297,6 +297,8 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       for (int k = 0; k < 9; ++k) fr[9 * n + k] = R[k];
     }
     c->h_coord = X;
+    c->h_sph = S;
+    c->dif.ready = c->dif.assembled = false;
     c->X.upload(X);
     c->S.upload(S);
     c->frames.upload(fr);
@@ -894,6 +896,19 @@ int m3_calculate_dt(m3_ctx *c, const double *U, double dt_adv_limit, double dtma
 int m3_sl_advect(m3_ctx *c, const double *T, const double *U, double dt, double T_cmb, double T_surf,
                  double *T_out) {
   return guard(c, [&] { m3::sl_advect(c, T, U, dt, T_cmb, T_surf, T_out); });
+}
+
+int m3_diffusion_mesh(m3_ctx *c, const uint32_t *EL2NOD, int64_t nel, int64_t nnodT, double theta_cone) {
+  return guard(c, [&] { m3::diffusion_mesh(c, EL2NOD, nel, nnodT, theta_cone); });
+}
+
+int m3_diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *ifix, const double *vfix,
+                   int64_t nfix, const m3_diffusion_opts *opts, m3_diffusion_info *info) {
+  return guard(c, [&] { m3::diffusion3d(c, T, kappa, dt, ifix, vfix, nfix, opts, info); });
+}
+
+int m3_diffusion_apply(m3_ctx *c, int32_t which, const double *x, double *y) {
+  return guard(c, [&] { m3::diffusion_apply(c, which, x, y); });
 }
 
 int m3_get_operator(m3_ctx *c, int32_t level, int64_t *rowptr, int32_t *col, double *val) {

@@ -213,6 +213,18 @@ struct m3_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[2];
   double prof_acc[2] = {0, 0};
   int64_t prof_n[2] = {0, 0};
+  // thermal diffusion on linear tetrahedra (a scalar field per node)
+  struct {
+    bool ready = false, assembled = false;
+    int64_t nel = 0, n = 0;
+    double theta_cone = 45.0;
+    DBuf<int32_t> el, slots;       // 4 nodes and 16 pattern slots per element
+    DBuf<uint8_t> cls;
+    Sell C;                        // pattern (1 value per slot); val = CC
+    DBuf<double> M, A;             // consistent mass and system values on C's slots
+    DBuf<double> ML, Adi, mask, vfix, T, rhs, x, r, rl, d, q, Aq;
+  } dif;
+  std::vector<double> h_sph;       // finest GCOORD_SPH row per node
   // semi-Lagrangian temperature advection on the finest P1 tetrahedra
   struct {
     bool ready = false;
@@ -286,4 +298,15 @@ void scal(hipStream_t s, double a, double *y, int64_t n);
 void build_mixed(m3_ctx *c);
 int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, int itmax,
               int precision, std::vector<double> *hist);
+// Jacobi-preconditioned CG on a scalar SELL operator (1 value per slot) whose
+// fixed rows/columns are identity: x and r (initial residual, zero at fixed
+// rows) in place; returns iterations, final RMS residual over nfree in *norm
+int pcg_diag_run(m3_ctx *c, SellView A, int64_t n, const double *Adi, double *x, double *r, double *rl,
+                 double *d, double *q, double *Aq, double nfree, double tol, int itmax, double *norm);
+void spmv_scalar(hipStream_t s, SellView A, int64_t n, const double *x, double *y);
+// thermal diffusion on linear tetrahedra
+void diffusion_mesh(m3_ctx *c, const uint32_t *EL2NOD, int64_t nel, int64_t nnodT, double theta_cone);
+void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *ifix, const double *vfix,
+                 int64_t nfix, const m3_diffusion_opts *o, m3_diffusion_info *info);
+void diffusion_apply(m3_ctx *c, int which, const double *x, double *y);
 }  // namespace m3

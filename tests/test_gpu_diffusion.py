@@ -1,0 +1,172 @@
+"""Thermal diffusion step (m3_diffusion_mesh / m3_diffusion3d) against the
+oracle (oracle/diffusion.py) on the same inputs.
+
+Tolerances: assembled operators 1e-12 relative (random probe vectors);
+temperatures after a step 1e-10 relative with the same PCG iteration count;
+a constant field with matching fixed values stays constant to 1e-12.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from tests.conftest import relerr
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def _fixed(m, n):
+    ifix = np.nonzero(np.isin(m.PointID[:n], (301, 306)))[0]
+    vfix = np.where(m.PointID[ifix] == 301, 1.0, 0.0)
+    return ifix, vfix
+
+
+@pytest.fixture(scope="module")
+def ctx0(c0):
+    from m3tet_sph_amd import api
+    log("GPU context c0 (diffusion)")
+    ctx = api.StokesContext(c0)
+    ctx.diffusion_mesh()
+    yield ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("nip", [4, 14])
+def test_operators_match_the_oracle(c0, ctx0, nip):
+    from oracle import diffusion as D
+    n = c0.nVnod
+    EL4 = c0.levels[0].EL2NOD[:, :4]
+    ctx0.diffusion3d(np.zeros(n), 0.37, 1.0, nip=nip)
+    Cv, Mv = D.element_matrices(c0.GCOORD, c0.GCOORD_SPH, EL4, 0.37, nip)
+    CC, MM = D.assemble(EL4, Cv, n), D.assemble(EL4, Mv, n)
+    ML = np.asarray(MM.sum(axis=1)).ravel()
+    rng = np.random.default_rng(nip)
+    for _ in range(3):
+        x = rng.standard_normal(n)
+        for which, ref in ((0, CC @ x), (1, MM @ x), (2, ML * x)):
+            got = ctx0.diffusion_apply(which, x)
+            log("diffusion operator", nip, which, relerr(got, ref))
+            assert relerr(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("theta,lumping,nip", [(1.0, "yes", 4), (0.5, "yes", 4), (1.0, "no", 4), (1.0, "yes", 11)])
+def test_step_matches_the_oracle(c0, ctx0, theta, lumping, nip):
+    from oracle import diffusion as D
+    n = c0.nVnod
+    EL4 = c0.levels[0].EL2NOD[:, :4]
+    rng = np.random.default_rng(7)
+    T0 = rng.random(n)
+    ifix, vfix = _fixed(c0, n)
+    T, info = ctx0.diffusion3d(T0, 1e-2, 0.5, ifix, vfix, theta=theta, lumping=lumping, nip=nip)
+    To, io = D.diffusion3d_p(T0, 1e-2, c0.GCOORD, c0.GCOORD_SPH, EL4, ifix, vfix, 0.5, theta=theta,
+                             lumping=lumping == "yes", nip=nip)
+    log("diffusion step", theta, lumping, nip, info["nit"], io["nit"], relerr(T, To))
+    assert info["nit"] == io["nit"]
+    assert relerr(info["norm_r0"], io["hist"][0]) < 1e-10
+    assert relerr(T, To) < 1e-10
+    assert np.all(T[ifix] == vfix)
+
+
+def test_split_p2_mesh_matches_the_oracle(c0):
+    """OPTS_D.eltype = 'linear' on the P2 mesh: every 10-node element split
+    into 8 linear tetrahedra (tetmesh_p2_to_p1.m), T on all P2 nodes."""
+    from m3tet_sph_amd import api, mesh as M
+    from oracle import diffusion as D
+    EL4, _ = M.p2_to_p1(c0.GCOORD, c0.levels[0].EL2NOD)
+    n = c0.nnod
+    ctx = api.StokesContext(c0)
+    try:
+        ctx.diffusion_mesh(EL4, n)
+        rng = np.random.default_rng(11)
+        T0 = rng.random(n)
+        ifix, vfix = _fixed(c0, n)
+        T, info = ctx.diffusion3d(T0, 3e-3, 1.0, ifix, vfix)
+        To, io = D.diffusion3d_p(T0, 3e-3, c0.GCOORD, c0.GCOORD_SPH, EL4, ifix, vfix, 1.0)
+        log("split P2 mesh", info["nit"], io["nit"], relerr(T, To))
+        assert info["nit"] == io["nit"]
+        assert relerr(T, To) < 1e-10
+    finally:
+        ctx.close()
+
+
+def test_edge_cases(c0, ctx0):
+    from m3tet_sph_amd import api
+    n = c0.nVnod
+    T0 = np.linspace(0.0, 1.0, n)
+    T, info = ctx0.diffusion3d(T0, 0.0, 1.0)            # kappa = 0: unchanged
+    assert np.array_equal(T, T0) and info["nit"] == 0
+    ifix, _ = _fixed(c0, n)
+    T, info = ctx0.diffusion3d(np.full(n, 0.3), 5e-2, 1.0, ifix, np.full(len(ifix), 0.3))
+    assert np.abs(T - 0.3).max() < 1e-12
+    allfix = np.arange(n)                                # every node fixed
+    T, info = ctx0.diffusion3d(T0, 1e-2, 1.0, allfix, T0[::-1])
+    assert np.array_equal(T, T0[::-1]) and info["nit"] == 0
+    with pytest.raises(api.M3Error, match="Unkown integration rule"):
+        ctx0.diffusion3d(T0, 1e-2, 1.0, nip=3)
+    with pytest.raises(api.M3Error, match="ifix out of range"):
+        ctx0.diffusion3d(T0, 1e-2, 1.0, [n], [0.0])
+    fresh = api.StokesContext(c0)
+    try:
+        with pytest.raises(api.M3Error, match="before m3_diffusion_mesh"):
+            fresh.diffusion3d(T0, 1e-2, 1.0)
+        with pytest.raises(api.M3Error, match="out of range"):
+            fresh.diffusion_mesh(c0.levels[0].EL2NOD[:, :4], 10)
+    finally:
+        fresh.close()
+
+
+def test_c1_step_matches_the_oracle():
+    """Full-size parity at c1 (43,554 vertices, 245,760 tetrahedra)."""
+    from m3tet_sph_amd import api, mesh as M
+    from oracle import diffusion as D
+    m = M.build_config("c1")
+    n = m.nVnod
+    EL4 = m.levels[0].EL2NOD[:, :4]
+    ctx = api.StokesContext(m)
+    try:
+        ctx.diffusion_mesh()
+        rng = np.random.default_rng(5)
+        T0 = rng.random(n)
+        ifix, vfix = _fixed(m, n)
+        T, info = ctx.diffusion3d(T0, 1e-3, 1.0, ifix, vfix)
+        To, io = D.diffusion3d_p(T0, 1e-3, m.GCOORD, m.GCOORD_SPH, EL4, ifix, vfix, 1.0)
+        log("c1 diffusion", info, io["nit"], relerr(T, To))
+        assert info["nit"] == io["nit"]
+        assert relerr(T, To) < 1e-10
+        # symmetry of the assembled conductivity through probes
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        a, b = x @ ctx.diffusion_apply(0, y), y @ ctx.diffusion_apply(0, x)
+        assert abs(a - b) < 1e-12 * (abs(a) + abs(b))
+    finally:
+        ctx.close()
+
+
+def test_thermal_advdiff_step(c0):
+    """thermal_advdiff_3d_p: diffusion dt/2, advection dt, diffusion dt/2."""
+    from m3tet_sph_amd import api, fields as F
+    from oracle import diffusion as D
+    inp = F.make_inputs(c0, visc_contrast=10.0)
+    ctx = api.StokesContext(c0)
+    c0._m3ctx = ctx
+    try:
+        ctx.set_bc(inp["ifix"], inp["vfix"])
+        U, P, _ = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], rtol_Pat=1e-6)
+        n = c0.nVnod
+        T0 = np.clip(np.random.default_rng(2).random(n), 0, 1)
+        ifix, vfix = _fixed(c0, n)
+        SET = {"OPTS_D": {"theta": 1.0, "lumping": "yes", "nip": 4}, "T_cmb": 1.0, "T_surf": 0.0}
+        dt = ctx.calculate_dt(U, 0.5)
+        VAR = api.thermal_advdiff_3d_p({"T": T0, "U": U}, c0, None, {"kappa": 1e-3}, SET, None,
+                                        {"ifix": ifix, "vfix": vfix}, dt)
+        # the same three stages from the oracle and the GPU advection
+        EL4 = c0.levels[0].EL2NOD[:, :4]
+        Ta, _ = D.diffusion3d_p(T0, 1e-3, c0.GCOORD, c0.GCOORD_SPH, EL4, ifix, vfix, dt / 2)
+        Tb = ctx.sl_advect(Ta, U, dt)
+        Tc, _ = D.diffusion3d_p(Tb, 1e-3, c0.GCOORD, c0.GCOORD_SPH, EL4, ifix, vfix, dt / 2)
+        log("thermal_advdiff", relerr(VAR["T"], Tc))
+        assert relerr(VAR["T"], Tc) < 1e-9
+    finally:
+        del c0._m3ctx
+        ctx.close()
