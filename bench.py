@@ -29,6 +29,14 @@ Forms:
 Prints one JSON line.  Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
                   [--form learned|fp64|mixed] [--no-cpu]
+
+``--path diffusion`` times the thermal diffusion step instead (not the
+headline metric): diffusion3d_p on the linear tetrahedra of --config
+(--grid vertex: the P1 vertex tetrahedra; p2split: every P2 element split
+into 8), element matrices + lumped mass + backward-Euler system with the
+CMB / surface temperatures fixed + Jacobi PCG to 1e-8, with the diffusivity
+set so that kappa dt / h_min^2 = 1; its cpu_baseline is the same step in the
+oracle.
 """
 import argparse
 import json
@@ -137,6 +145,63 @@ def roofline(ctx, mesh, api, run_once, form, config):
                 avg_us=round(avg * 1e6, 2), bytes_per_launch=int(bpl))
 
 
+def diffusion_bench(args):
+    """One JSON line for the thermal diffusion step (see the module docstring)."""
+    from m3tet_sph_amd import mesh as M, api
+
+    t0 = time.perf_counter()
+    m = M.build_config(args.config)
+    if args.grid == "vertex":
+        EL4, n = m.levels[0].EL2NOD[:, :4], m.nVnod
+    else:
+        EL4, _ = M.p2_to_p1(m.GCOORD, m.levels[0].EL2NOD)
+        n = m.nnod
+    X = m.GCOORD[:n]
+    h_min = min(float(np.min(np.linalg.norm(X[EL4[:, i]] - X[EL4[:, j]], axis=1)))
+                for i in range(4) for j in range(i + 1, 4))
+    dt = 1.0
+    kappa = h_min ** 2 / dt
+    ctx = api.StokesContext(m)
+    ctx.diffusion_mesh(EL4, n)
+    T0 = np.random.default_rng(0).random(n)
+    ifix = np.nonzero(np.isin(m.PointID[:n], (301, 306)))[0]
+    vfix = np.where(m.PointID[ifix] == 301, 1.0, 0.0)
+    log(f"setup {time.perf_counter() - t0:.1f}s {args.config} {args.grid}: {len(EL4)} tets, {n} nodes, "
+        f"kappa {kappa:.3e}")
+    for _ in range(args.warmup):
+        ctx.diffusion3d(T0, kappa, dt, ifix, vfix)
+    infos = []
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        T, info = ctx.diffusion3d(T0, kappa, dt, ifix, vfix)
+        infos.append(info)
+    sec = (time.perf_counter() - t) / args.steps
+    pairs = np.concatenate([EL4[:, i].astype(np.int64) * n + EL4[:, j] for i in range(4) for j in range(4)])
+    nnz = int(np.unique(pairs).size)
+    del pairs
+    out = {"metric": "thermal diffusion steps/s (assembly + Jacobi PCG to 1e-8), 1 GPU",
+           "value": round(1.0 / sec, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * sec, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"{args.config} {args.grid}: {len(EL4)} linear tetrahedra, {n} nodes, "
+                                  f"kappa dt / h_min^2 = 1, random initial temperature"},
+           "nit": infos[-1]["nit"],
+           "t_assemble_ms": round(1e3 * float(np.mean([i["t_assemble"] for i in infos])), 3),
+           "t_solve_ms": round(1e3 * float(np.mean([i["t_solve"] for i in infos])), 3),
+           "nnz": nnz, "spmv_bytes": nnz * 12 + n * 16}
+    if not args.no_cpu:
+        from oracle import diffusion as OD
+        t = time.perf_counter()
+        To, io = OD.diffusion3d_p(T0, kappa, m.GCOORD, m.GCOORD_SPH, EL4, ifix, vfix, dt)
+        tc = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": round(1.0 / tc, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": "the same step in the oracle (NumPy / SciPy, one process)",
+                               "nit": io["nit"]}
+        out["max_abs_diff_vs_oracle"] = float(np.abs(T - To).max())
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,12 +213,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--visc-contrast", type=float, default=None,
                     help="default per config: c0 1e3, c1 1e4, c2 1e5, c4 1e6")
+    ap.add_argument("--path", default="stokes", choices=["stokes", "diffusion"])
+    ap.add_argument("--grid", default="vertex", choices=["vertex", "p2split"])
     args = ap.parse_args()
     vc = args.visc_contrast or {"c0": 1e3, "c1": 1e4, "c2": 1e5, "c4": 1e6}.get(args.config, 1e4)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on exactly one GPU (the Stokes solve is single-device; "
               "multi-GPU sharding is a design note only). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.path == "diffusion":
+        diffusion_bench(args)
+        return
 
     from m3tet_sph_amd import mesh as M, fields as F, api
 

@@ -288,8 +288,9 @@ int m3_diffusion3d(m3_ctx *ctx, double *T, double kappa, double dt, const int64_
                    int64_t nfix, const m3_diffusion_opts *opts, m3_diffusion_info *info);
 
 /* Operators of the last m3_diffusion3d call for parity checks: y = CC x
- * (which 0), y = MM x (which 1, the consistent mass matrix), y = MM_lumped x
- * (which 2).  x, y: nnodT.  No reference counterpart (diagnostics). */
+ * (which 0), y = MM x (which 1, the consistent mass matrix, assembled only by
+ * a step with lumping = 0), y = MM_lumped x (which 2).  x, y: nnodT.  No
+ * reference counterpart (diagnostics). */
 int m3_diffusion_apply(m3_ctx *ctx, int32_t which, const double *x, double *y);
 
 #ifdef __cplusplus

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_diff_elem(DiffArgs a) {
     for (int q = 0; q < 4; ++q) {
       const int lo = p < q ? p : q, hi = p < q ? q : p;
       atomicAdd(a.C + slot[4 * p + q], C[lo][hi]);
-      atomicAdd(a.M + slot[4 * p + q], M[lo][hi]);
+      if (a.M) atomicAdd(a.M + slot[4 * p + q], M[lo][hi]);
       ml += M[lo][hi];
     }
     atomicAdd(a.ML + n[p], ml);
@@ -397,9 +397,10 @@ void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *i
   hipStream_t s = c->stream;
   const int64_t n = D.n;
   auto t0 = clk::now();
-  // element matrices
+  // element matrices (the consistent mass only when it is used: lumping 'no')
+  const bool lumped = o->lumping != 0;
   D.C.val.zero(s);
-  D.M.zero(s);
+  if (!lumped) D.M.zero(s);
   D.ML.zero(s);
   c->errflag.zero(s);
   DiffArgs a{};
@@ -412,7 +413,7 @@ void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *i
   a.kappa = kappa;
   a.ip = ipt;
   a.C = D.C.val.p;
-  a.M = D.M.p;
+  a.M = lumped ? nullptr : D.M.p;
   a.ML = D.ML.p;
   a.err = c->errflag.p;
   hipLaunchKernelGGL(k_diff_elem, dim3(nblk(D.nel)), dim3(256), 0, s, a);
@@ -423,6 +424,7 @@ void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *i
   if (err & 1) throw M3Error(M3_E_NUMERIC, "negative Jacobi");
   if (err & 2) throw M3Error(M3_E_NUMERIC, "integration point has theta = 0 (det(J_CS) = Inf)");
   D.assembled = true;
+  D.have_M = !lumped;
   // fixed nodes
   std::vector<double> mask(n, 1.0), vf(n, 0.0);
   for (int64_t k = 0; k < nfix; ++k) {
@@ -438,7 +440,7 @@ void diffusion3d(m3_ctx *c, double *T, double kappa, double dt, const int64_t *i
   D.T.upload(T, n);
   const double t_asm = std::chrono::duration<double>(clk::now() - t0).count();
   auto t1 = clk::now();
-  SysArgs sa{D.C.view(), D.M.p, D.ML.p, o->lumping ? 1 : 0, n, dt, o->theta, D.mask.p, D.vfix.p, D.T.p};
+  SysArgs sa{D.C.view(), D.M.p, D.ML.p, lumped ? 1 : 0, n, dt, o->theta, D.mask.p, D.vfix.p, D.T.p};
   hipLaunchKernelGGL(k_diff_rhs, dim3(nblk(n)), dim3(256), 0, s, sa, D.rhs.p);
   hipLaunchKernelGGL(k_diff_sys, dim3(nblk(n)), dim3(256), 0, s, sa, D.A.p, D.Adi.p, D.x.p);
   SellView Av = D.C.view();
@@ -478,6 +480,7 @@ void diffusion_apply(m3_ctx *c, int which, const double *x, double *y) {
   M3_REQUIRE(D.assembled, M3_E_STATE, "no diffusion operators yet (call m3_diffusion3d)");
   M3_REQUIRE(x && y, M3_E_ARG, "null argument");
   M3_REQUIRE(which >= 0 && which <= 2, M3_E_ARG, "which must be 0, 1 or 2");
+  M3_REQUIRE(which != 1 || D.have_M, M3_E_STATE, "consistent mass not assembled (last step used lumping)");
   hipStream_t s = c->stream;
   const int64_t n = D.n;
   D.d.upload(x, n);

@@ -215,7 +215,7 @@ struct m3_ctx {
   int64_t prof_n[2] = {0, 0};
   // thermal diffusion on linear tetrahedra (a scalar field per node)
   struct {
-    bool ready = false, assembled = false;
+    bool ready = false, assembled = false, have_M = false;
     int64_t nel = 0, n = 0;
     double theta_cone = 45.0;
     DBuf<int32_t> el, slots;       // 4 nodes and 16 pattern slots per element

@@ -1,6 +1,7 @@
 """Consecutive time steps of the convection loop with the Stokes hot path:
 solve (mixed-precision V-cycle, started from the previous step), CFL time
-step, semi-Lagrangian temperature update; optionally, at every step, a
+step, semi-Lagrangian temperature update (with --kappa: diffusion over dt/2,
+advection over dt, diffusion over dt/2); optionally, at every step, a
 second solve of the same state started from the warm-start network, so the
 two starts are compared on identical inputs.
 
@@ -35,6 +36,9 @@ def main(argv=None):
     ap.add_argument("--Ra", type=float, default=1e5)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--net", default=None, help="warm-start network (.npz) to compare against")
+    ap.add_argument("--kappa", type=float, default=0.0,
+                    help="thermal diffusivity: > 0 adds diffusion over dt/2 before and after the "
+                         "advection (thermal_advdiff_3d_p.m:13-52)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
@@ -56,6 +60,10 @@ def main(argv=None):
     nV = m.nVnod
     Tv = fields.temperature(m, seed=2, plume=plume)[:nV].copy()
     U = P = Tp = None
+    if a.kappa > 0:
+        ctx.diffusion_mesh()
+        tfix = np.nonzero(np.isin(m.PointID[:nV], (301, 306)))[0]
+        tval = np.where(m.PointID[tfix] == 301, 1.0, 0.0)
     rows = []
     rP_cold = None
     for k in range(a.steps + 1):
@@ -88,7 +96,14 @@ def main(argv=None):
         rows.append(row)
         log(f"[step {k}] " + json.dumps({kk: vv for kk, vv in row.items() if kk != "step"}))
         Tp = Tv
-        Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
+        if a.kappa > 0:
+            t = time.perf_counter()
+            Tv, d1 = ctx.diffusion3d(Tv, a.kappa, dt / 2, tfix, tval)
+            Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
+            Tv, d2 = ctx.diffusion3d(Tv, a.kappa, dt / 2, tfix, tval)
+            row["thermal"] = dict(s=time.perf_counter() - t, nit=[d1["nit"], d2["nit"]])
+        else:
+            Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
     # sweep bandwidth of one more solve of the last state
     api.profile(ctx, True)
     ctx.flow3d(V, Dn, phys, P0=P, U0=U, precision=a.form, rtol_U_final=0.5 * a.tol,
@@ -107,6 +122,9 @@ def main(argv=None):
                           nitZ=float(np.mean([r["prev"]["nitZ"] for r in steps]))),
                 max_rel_res=float(max(r["rel_res"] for r in rows)), sweep_GBs=gbs,
                 sweep_avg_us=t_ / max(n_, 1) * 1e6, cold_step0=rows[0]["cold"])
+    if a.kappa > 0:
+        summ["thermal"] = dict(kappa=a.kappa, s=float(np.mean([r["thermal"]["s"] for r in rows])),
+                               nit=float(np.mean([sum(r["thermal"]["nit"]) for r in rows])))
     if net is not None:
         summ["net"] = dict(s=float(np.mean([r["net"]["s"] for r in steps])),
                            nitP=float(np.mean([r["net"]["nitP"] for r in steps])),

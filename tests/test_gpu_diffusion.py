@@ -38,7 +38,7 @@ def test_operators_match_the_oracle(c0, ctx0, nip):
     from oracle import diffusion as D
     n = c0.nVnod
     EL4 = c0.levels[0].EL2NOD[:, :4]
-    ctx0.diffusion3d(np.zeros(n), 0.37, 1.0, nip=nip)
+    ctx0.diffusion3d(np.zeros(n), 0.37, 1.0, nip=nip, lumping="no")
     Cv, Mv = D.element_matrices(c0.GCOORD, c0.GCOORD_SPH, EL4, 0.37, nip)
     CC, MM = D.assemble(EL4, Cv, n), D.assemble(EL4, Mv, n)
     ML = np.asarray(MM.sum(axis=1)).ravel()
@@ -103,6 +103,8 @@ def test_edge_cases(c0, ctx0):
     allfix = np.arange(n)                                # every node fixed
     T, info = ctx0.diffusion3d(T0, 1e-2, 1.0, allfix, T0[::-1])
     assert np.array_equal(T, T0[::-1]) and info["nit"] == 0
+    with pytest.raises(api.M3Error, match="consistent mass not assembled"):
+        ctx0.diffusion_apply(1, T0)                      # the last step above was lumped
     with pytest.raises(api.M3Error, match="Unkown integration rule"):
         ctx0.diffusion3d(T0, 1e-2, 1.0, nip=3)
     with pytest.raises(api.M3Error, match="ifix out of range"):
