@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from . import api, fields, learned as L, mesh as M
+from . import api, fields, learned as L, matio, mesh as M
 
 
 def log(*a):
@@ -40,6 +40,10 @@ def main(argv=None):
                     help="thermal diffusivity: > 0 adds diffusion over dt/2 before and after the "
                          "advection (thermal_advdiff_3d_p.m:13-52)")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--snapshots", type=int, default=0,
+                    help="every N steps write <out>/<config>_Data_<n>.mat (write_output_file.m) and "
+                         "alternate restart files <config>_RestartFile{1,2}.mat (write_restart_file.m)")
+    ap.add_argument("--restart", default=None, help="start from the VAR.T of a restart / output .mat file")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     vc = a.visc_contrast or {"c0": 1e3, "c1": 1e4, "c2": 1e5, "c4": 1e6}.get(a.config, 1e4)
@@ -59,6 +63,16 @@ def main(argv=None):
     phys = fields.physics(a.Ra)
     nV = m.nVnod
     Tv = fields.temperature(m, seed=2, plume=plume)[:nV].copy()
+    time_ = 0.0
+    if a.restart:
+        R, time_, istep0, _ = matio.load_restart_file(a.restart)
+        if len(R["T"]) not in (nV, m.nnod):
+            raise SystemExit(f"{a.restart}: VAR.T has {len(R['T'])} entries, mesh has {m.nnod} nodes")
+        Tv = np.asarray(R["T"], dtype=np.float64)[:nV].copy()
+        log(f"[restart] {a.restart}: step {istep0}, time {time_}")
+    frames = fields.rotation_frames(m.GCOORD) if a.snapshots else None
+    SET, COMM = {"outdir": a.out}, {"prefix": a.config}
+    iplot = 0
     U = P = Tp = None
     if a.kappa > 0:
         ctx.diffusion_mesh()
@@ -95,6 +109,15 @@ def main(argv=None):
         row["dt"] = dt
         rows.append(row)
         log(f"[step {k}] " + json.dumps({kk: vv for kk, vv in row.items() if kk != "step"}))
+        if a.snapshots and k % a.snapshots == 0:
+            Uxyz = np.einsum("nij,nj->ni", frames, U.reshape(-1, 3))
+            VAR = dict(T=L.nodal_temperature(m, Tv, parent), Ux=Uxyz[:, 0].copy(), Uy=Uxyz[:, 1].copy(),
+                       Uz=Uxyz[:, 2].copy(), Uth=U[0::3].copy(), Uph=U[1::3].copy(), Ur=U[2::3].copy(), P=P,
+                       Visc=V, Dens=Dn)
+            iplot += 1
+            matio.write_output_file(SET, VAR, COMM, time_, k, dt, iplot)
+            matio.write_restart_file(SET, VAR, COMM, k, iplot, time_, dt, 1 + (iplot - 1) % 2)
+        time_ += dt
         Tp = Tv
         if a.kappa > 0:
             t = time.perf_counter()

@@ -2,6 +2,8 @@
 and the time step on the device against oracle/material.py, assembly from the
 device-resident fields, and a short time loop (material law -> solve -> dt ->
 semi-Lagrangian step) that must match the same loop driven from host arrays."""
+import os
+
 import numpy as np
 import pytest
 
@@ -86,3 +88,24 @@ def test_short_time_loop_on_device_fields(c0, c0_inputs, ctx):
         runs.append(hist)
     for (U1, P1, T1, n1), (U2, P2, T2, n2) in zip(*runs):
         assert relerr(U2, U1) < 1e-9 and relerr(P2, P1) < 1e-9 and relerr(T2, T1) < 1e-12
+
+
+def test_timeloop_driver_snapshots_and_restart(tmp_path):
+    """The time-loop driver on c0 with diffusion, snapshot / restart files in
+    the reference's MAT layout, and a restart from the last snapshot."""
+    import json
+    from m3tet_sph_amd import matio, timeloop
+    out = str(tmp_path)
+    timeloop.main(["--config", "c0", "--steps", "3", "--kappa", "1e2", "--snapshots", "2", "--out", out])
+    summ = json.load(open(os.path.join(out, "timeloop_c0.json")))["summary"]
+    assert summ["max_rel_res"] <= 1e-6 and summ["thermal"]["nit"] > 0
+    files = sorted(os.listdir(out))
+    assert "c0_Data_0001.mat" in files and "c0_Data_0002.mat" in files
+    assert "c0_RestartFile1.mat" in files and "c0_RestartFile2.mat" in files
+    VAR, t, istep, iplot = matio.load_restart_file(os.path.join(out, "c0_RestartFile2.mat"))
+    assert istep == 2 and iplot == 2 and t > 0
+    assert set(VAR) >= {"T", "Ux", "Uy", "Uz", "Uth", "Uph", "Ur", "P", "Visc", "Dens"}
+    out2 = str(tmp_path / "r")
+    timeloop.main(["--config", "c0", "--steps", "1", "--restart", os.path.join(out, "c0_RestartFile2.mat"),
+                   "--out", out2])
+    assert os.path.exists(os.path.join(out2, "timeloop_c0.json"))
