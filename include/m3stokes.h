@@ -293,6 +293,14 @@ int m3_diffusion3d(m3_ctx *ctx, double *T, double kappa, double dt, const int64_
  * reference counterpart (diagnostics). */
 int m3_diffusion_apply(m3_ctx *ctx, int32_t which, const double *x, double *y);
 
+/* Mesh set-up: unique undirected bars in the order of their first occurrence,
+ * find_unique_bars (mfiles_SPH/tetmesh_p1_to_p2_sph.m:238-254), on HIP device
+ * `device` (no context).  bars: n x 2 node indices (row-major, >= 0, either
+ * orientation).  Out: *nu unique bars; first[u] = position of the first
+ * occurrence of unique bar u (first needs room for n entries); bar_of[i] =
+ * unique index of bar i.  Returns M3_E_ARG for negative node indices. */
+int m3_unique_bars(int32_t device, const int32_t *bars, int64_t n, int64_t *first, int64_t *bar_of, int64_t *nu);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,6 +150,27 @@ LIB.m3_diffusion_apply.argtypes = [C.c_void_p, C.c_int32, D, D]
 for _f in ("m3_diffusion_mesh", "m3_diffusion3d", "m3_diffusion_apply"):
     getattr(LIB, _f).restype = C.c_int
 
+LIB.m3_unique_bars.argtypes = [C.c_int32, I32, C.c_int64, I64, I64, I64]
+LIB.m3_unique_bars.restype = C.c_int
+
+
+def unique_bars(bars, device=0):
+    """find_unique_bars (tetmesh_p1_to_p2_sph.m:238-254) on the GPU: (first,
+    bar_of) with first[u] the position of unique bar u's first occurrence (in
+    order of appearance) and bar_of[i] the unique index of bar i."""
+    b = np.ascontiguousarray(bars, dtype=np.int32)
+    if b.ndim != 2 or b.shape[1] != 2:
+        raise M3Error(1, "expecting a nx2 matrix")
+    n = len(b)
+    first = np.empty(max(n, 1), dtype=np.int64)
+    bar_of = np.empty(max(n, 1), dtype=np.int64)
+    nu = C.c_int64(0)
+    rc = LIB.m3_unique_bars(int(device), _p(b, I32), n, _p(first, I64), _p(bar_of, I64), C.byref(nu))
+    if rc:
+        raise M3Error(rc, "m3_unique_bars failed")
+    return first[:nu.value].copy(), bar_of[:n].copy()
+
+
 LIB.m3_rock_prop.argtypes = [C.c_void_p, D, C.POINTER(RockParams), D, D]
 LIB.m3_calculate_dt.argtypes = [C.c_void_p, D, C.c_double, C.c_double, D]
 LIB.m3_rock_prop.restype = C.c_int

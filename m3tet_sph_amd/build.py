@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libm3stokes.so")
 SOURCES = ["m3_api.hip", "k_assembly.hip", "k_sparse.hip", "k_chol.hip", "k_advect.hip", "k_gnn.hip", "k_material.hip",
-           "k_diffusion.hip"]
+           "k_diffusion.hip", "k_mesh.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

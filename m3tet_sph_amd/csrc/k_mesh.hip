@@ -1,0 +1,199 @@
+// Mesh set-up on the GPU: the unique-bar numbering of the P1 -> P2 promotion,
+// find_unique_bars (mfiles_SPH/tetmesh_p1_to_p2_sph.m:238-254): unique
+// undirected bars ordered by the position of their first occurrence, and for
+// every input bar the index of its unique bar.  It is the dominant set-up
+// cost of the finer levels (17.7 M bars on the c4 finest level).
+//
+// No sort: every bar's key (min node, max node) is inserted into an
+// open-addressing hash table (linear probing, load factor <= 1/2) whose slot
+// keeps the smallest position that holds the key (atomicMin, so the result
+// does not depend on the order of the atomics).  A bar is a first occurrence
+// when its slot's position is its own; an exclusive prefix sum of those flags
+// in position order gives each unique bar its rank, i.e. the first-appearance
+// order, and every bar takes the rank of its slot's first position.
+#include "m3_internal.h"
+
+#include <climits>
+
+namespace {
+
+constexpr unsigned long long EMPTY = ~0ull;
+constexpr int SB = 1024;  // scan block
+
+__device__ inline unsigned long long mix(unsigned long long k) {
+  // 64-bit finaliser (MurmurHash3 fmix64)
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return k;
+}
+
+__device__ inline unsigned long long bar_key(const int32_t *bars, int64_t i) {
+  const unsigned a = (unsigned)bars[2 * i], b = (unsigned)bars[2 * i + 1];
+  const unsigned lo = a < b ? a : b, hi = a < b ? b : a;
+  return ((unsigned long long)lo << 32) | hi;
+}
+
+__global__ void k_fill_u64(int64_t n, unsigned long long *x, unsigned long long v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// insert every bar; slot[i] = its table slot; the probe count is bounded by
+// the table size (the table is at least twice the number of bars, so a free
+// slot always exists and the loop ends long before the bound)
+__global__ void k_insert(int64_t n, const int32_t *bars, unsigned long long mask, unsigned long long *keys,
+                         unsigned long long *firstpos, int64_t *slot, int *err) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long k = bar_key(bars, i);
+  unsigned long long s = mix(k) & mask;
+  for (unsigned long long probe = 0; probe <= mask; ++probe) {
+    const unsigned long long prev = atomicCAS(&keys[s], EMPTY, k);
+    if (prev == EMPTY || prev == k) {
+      atomicMin(&firstpos[s], (unsigned long long)i);
+      slot[i] = (int64_t)s;
+      return;
+    }
+    s = (s + 1) & mask;
+  }
+  atomicOr(err, 1);
+  slot[i] = -1;
+}
+
+// flag[i] = 1 when bar i is the first occurrence of its key
+__global__ void k_first_flag(int64_t n, const int64_t *slot, const unsigned long long *firstpos, int64_t *flag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = (slot[i] >= 0 && firstpos[slot[i]] == (unsigned long long)i) ? 1 : 0;
+}
+
+// exclusive scan, phase 1: per-block sums
+__global__ __launch_bounds__(SB) void k_scan_sums(int64_t n, const int64_t *x, int64_t *sums) {
+  __shared__ int64_t sh[SB];
+  const int64_t i = blockIdx.x * (int64_t)SB + threadIdx.x;
+  sh[threadIdx.x] = i < n ? x[i] : 0;
+  __syncthreads();
+  for (int st = SB / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = sh[0];
+}
+
+// in-block exclusive scan of SB values (Hillis-Steele on LDS)
+__device__ inline int64_t block_exscan(int64_t v, int64_t *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < SB; off <<= 1) {
+    const int64_t t = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  return sh[threadIdx.x] - v;
+}
+
+// phase 2: exclusive scan of the block sums by one workgroup, chunk by chunk;
+// total written to sums[nb]
+__global__ __launch_bounds__(SB) void k_scan_top(int64_t nb, int64_t *sums) {
+  __shared__ int64_t sh[SB];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += SB) {
+    const int64_t i = c0 + threadIdx.x;
+    const int64_t v = i < nb ? sums[i] : 0;
+    const int64_t ex = block_exscan(v, sh);
+    const int64_t tot = sh[SB - 1];
+    __syncthreads();
+    if (i < nb) sums[i] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+// phase 3: rank[i] = exclusive prefix of flag at i
+__global__ __launch_bounds__(SB) void k_scan_apply(int64_t n, const int64_t *flag, const int64_t *sums,
+                                                    int64_t *rank) {
+  __shared__ int64_t sh[SB];
+  const int64_t i = blockIdx.x * (int64_t)SB + threadIdx.x;
+  const int64_t v = i < n ? flag[i] : 0;
+  const int64_t ex = block_exscan(v, sh);
+  if (i < n) rank[i] = sums[blockIdx.x] + ex;
+}
+
+// bar_of[i] = rank of the first occurrence of bar i's key; first[rank] = i
+__global__ void k_bar_out(int64_t n, const int64_t *slot, const unsigned long long *firstpos, const int64_t *flag,
+                          const int64_t *rank, int64_t *bar_of, int64_t *first) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (slot[i] < 0) {  // table overflow, reported through err
+    bar_of[i] = -1;
+    return;
+  }
+  bar_of[i] = rank[(int64_t)firstpos[slot[i]]];
+  if (flag[i]) first[rank[i]] = i;
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+extern "C" int m3_unique_bars(int32_t device, const int32_t *bars, int64_t n, int64_t *first, int64_t *bar_of,
+                              int64_t *nu) {
+  if (!bars || !first || !bar_of || !nu || n < 0) return M3_E_ARG;
+  *nu = 0;
+  if (n == 0) return M3_OK;
+  if (n > ((int64_t)1 << 40)) return M3_E_ARG;
+  try {
+    HIPCHECK(hipSetDevice(device));
+    for (int64_t i = 0; i < 2 * n; ++i)
+      if (bars[i] < 0) return M3_E_ARG;
+    struct Stream {
+      hipStream_t s = nullptr;
+      ~Stream() {
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } st;
+    HIPCHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    hipStream_t s = st.s;
+    unsigned long long tsize = 1;
+    while (tsize < 2ull * (unsigned long long)n) tsize <<= 1;
+    DBuf<int32_t> dbars;
+    DBuf<unsigned long long> keys, firstpos;
+    DBuf<int64_t> slot, flag, rank, sums, dbar_of, dfirst;
+    DBuf<int> err;
+    dbars.upload(bars, 2 * n);
+    keys.alloc(tsize);
+    firstpos.alloc(tsize);
+    for (auto *b : {&slot, &flag, &rank, &dbar_of, &dfirst}) b->alloc(n);
+    const int64_t nb = (n + SB - 1) / SB;
+    sums.alloc(nb + 1);
+    err.alloc(1);
+    err.zero(s);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(tsize)), dim3(256), 0, s, (int64_t)tsize, keys.p, EMPTY);
+    hipLaunchKernelGGL(k_fill_u64, dim3(nblk(tsize)), dim3(256), 0, s, (int64_t)tsize, firstpos.p, EMPTY);
+    hipLaunchKernelGGL(k_insert, dim3(nblk(n)), dim3(256), 0, s, n, dbars.p, tsize - 1, keys.p, firstpos.p, slot.p,
+                       err.p);
+    hipLaunchKernelGGL(k_first_flag, dim3(nblk(n)), dim3(256), 0, s, n, slot.p, firstpos.p, flag.p);
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nb), dim3(SB), 0, s, n, flag.p, sums.p);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SB), 0, s, nb, sums.p);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SB), 0, s, n, flag.p, sums.p, rank.p);
+    hipLaunchKernelGGL(k_bar_out, dim3(nblk(n)), dim3(256), 0, s, n, slot.p, firstpos.p, flag.p, rank.p,
+                       dbar_of.p, dfirst.p);
+    HIPCHECK(hipGetLastError());
+    int e = 0;
+    int64_t total = 0;
+    HIPCHECK(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(&total, sums.p + nb, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    M3_REQUIRE(e == 0, M3_E_NUMERIC, "bar hash table overflow");
+    dbar_of.download(bar_of, n);
+    dfirst.download(first, total);
+    *nu = total;
+    return M3_OK;
+  } catch (const M3Error &e) {
+    return e.code;
+  }
+}

@@ -73,13 +73,18 @@ def sph2cart(S):
                      r * np.cos(th)], axis=1)
 
 
-def unique_first_order(pairs):
+def unique_first_order(pairs, device=None):
     """Unique unordered rows in order of first appearance.
 
     Mirrors ``find_unique_bars`` / ``unique_keep_order``: returns the unique
     rows (orientation of their first occurrence) and, for every input row, the
-    index of its unique row.
+    index of its unique row.  With ``device`` (a HIP device index) two-column
+    rows are numbered on that GPU (``m3_unique_bars``); the result is the same.
     """
+    if device is not None and pairs.ndim == 2 and pairs.shape[1] == 2:
+        from . import api
+        first, bar_of = api.unique_bars(pairs, device)
+        return pairs[first], bar_of
     s = np.sort(pairs, axis=1).astype(np.int64)
     base = int(s.max()) + 1 if s.size else 1
     if s.shape[1] == 2:
@@ -188,14 +193,14 @@ def classify(SPH4, EL4, theta_cone=THETA_CONE_DEG):
     ), inc
 
 
-def p1_to_p2_sph(X4, SPH4, EL4, pid4, r_cmb, r_surf, theta_cone=THETA_CONE_DEG):
+def p1_to_p2_sph(X4, SPH4, EL4, pid4, r_cmb, r_surf, theta_cone=THETA_CONE_DEG, device=None):
     """P1 -> curved P2 (tetmesh_p1_to_p2_sph.m:183-232, 512-634)."""
     nel = len(EL4)
     n4 = len(SPH4)
     classes, inc = classify(SPH4, EL4, theta_cone)
     tc = theta_cone * PI / 180.0
     edges = EL4[:, TET_EDGES].reshape(-1, 2)
-    bars, bar_of_edge = unique_first_order(edges)
+    bars, bar_of_edge = unique_first_order(edges, device)
     el2bar = bar_of_edge.reshape(nel, 6)
     # default: straight in the original spherical frame, phi unwrapped
     th_e = SPH4[bars, 0]
@@ -441,12 +446,13 @@ class ShellMesh:
 
 
 def build_shell(level, nlayer, nmg, r_in=0.546, r_out=1.0,
-                theta_cone=THETA_CONE_DEG):
-    """Base icosahedral shell (``level`` x ``nlayer``) refined nmg-1 times."""
+                theta_cone=THETA_CONE_DEG, device=None):
+    """Base icosahedral shell (``level`` x ``nlayer``) refined nmg-1 times;
+    with ``device`` the bar numbering of every level runs on that GPU."""
     X4, EL4, pid4 = shell_p1(level, nlayer, r_in, r_out)
     SPH4 = cart2sph(X4)
     X, SPH, EL10, pid, classes, inc, _ = p1_to_p2_sph(X4, SPH4, EL4, pid4, r_in, r_out,
-                                                      theta_cone)
+                                                      theta_cone, device)
     levels = [Level(EL10, len(X), len(X4))]
     isplits = []
     for _ in range(nmg - 1):
@@ -454,7 +460,7 @@ def build_shell(level, nlayer, nmg, r_in=0.546, r_out=1.0,
         isplits.insert(0, isplit)
         X4, SPH4, EL4, pid4 = X, SPH, child, pid
         X, SPH, EL10, pid, classes, inc, _ = p1_to_p2_sph(X4, SPH4, EL4, pid4, r_in, r_out,
-                                                          theta_cone)
+                                                          theta_cone, device)
         levels.insert(0, Level(EL10, len(X), len(X4)))
     X20 = cubic_iso_coords(X4, SPH4, EL10[:, :4], classes["in_cone_iso"], theta_cone)
     return ShellMesh(levels=levels, GCOORD=X, GCOORD_SPH=SPH, PointID=pid, classes=classes,
@@ -472,5 +478,5 @@ CONFIGS = {
 }
 
 
-def build_config(name):
-    return build_shell(**CONFIGS[name])
+def build_config(name, device=None):
+    return build_shell(**CONFIGS[name], device=device)
