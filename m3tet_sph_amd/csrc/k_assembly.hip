@@ -576,6 +576,33 @@ __global__ __launch_bounds__(64 * GK_ROWS) void k_gatherK(SellView K, int64_t nr
   }
 }
 
+// slot of every element block in K (10x10 at [10a+b]), G (10x4 at
+// [100+4a+v]), GT (4x10 at [140+4a+v]) and M (4x4 at [180+4v+w]); one thread
+// per (element, block), a binary search in the sorted row
+__global__ void k_slots(int64_t nel, const int32_t *el2nod, SellView K, SellView G, SellView GT, SellView M,
+                        int32_t *slots, int *err) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nel * ASM_SLOTS) return;
+  const int64_t e = t / ASM_SLOTS;
+  const int k = (int)(t - e * ASM_SLOTS);
+  const int32_t *n = el2nod + 10 * e;
+  int64_t pos;
+  if (k < 100) {
+    pos = sell_find(K.slice_ptr, K.rowlen, K.col, n[k / 10], n[k % 10]);
+  } else if (k < 140) {
+    const int j = k - 100;
+    pos = sell_find(G.slice_ptr, G.rowlen, G.col, n[j / 4], n[j % 4]);
+  } else if (k < 180) {
+    const int j = k - 140;
+    pos = sell_find(GT.slice_ptr, GT.rowlen, GT.col, n[j % 4], n[j / 4]);
+  } else {
+    const int j = k - 180;
+    pos = sell_find(M.slice_ptr, M.rowlen, M.col, n[j / 4], n[j % 4]);
+  }
+  if (pos < 0) atomicOr(err, 8);
+  slots[t] = (int32_t)pos;
+}
+
 // identity at fixed dofs on the diagonal blocks, inverse diagonal
 __global__ void k_fix_diag(SellView K, const double *mask, int64_t nnod, double *dinv) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -695,6 +722,14 @@ void launch_gather_K(m3_ctx *c) {
   LevelDev &f = c->L[0];
   hipLaunchKernelGGL(k_gatherK, dim3((unsigned)((f.nnod + GK_ROWS - 1) / GK_ROWS)), dim3(64 * GK_ROWS), 0,
                      c->stream, f.K.view(), f.nnod, c->inc_ptr.p, c->inc.p, c->asm_slots.p, f.Kel.p);
+  HIPCHECK(hipGetLastError());
+}
+
+void launch_slots(m3_ctx *c) {
+  LevelDev &f = c->L[0];
+  const int64_t nt = f.nel * ASM_SLOTS;
+  hipLaunchKernelGGL(k_slots, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, f.nel, f.el2nod.p,
+                     f.K.view(), c->G.view(), c->GT.view(), c->M.view(), c->asm_slots.p, c->errflag.p);
   HIPCHECK(hipGetLastError());
 }
 

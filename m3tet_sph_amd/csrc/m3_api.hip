@@ -6,8 +6,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <numeric>
+#include <unordered_map>
 
 #include "m3_internal.h"
 
@@ -50,7 +53,10 @@ double secs(clk::time_point a, clk::time_point b) {
   return std::chrono::duration<double>(b - a).count();
 }
 
-// rows = nodes 0..nrow-1; for each element, row nodes rsel[] x col nodes csel[]
+// rows = nodes 0..nrow-1; for each element, row nodes rsel[] x col nodes csel[].
+// Sorted, duplicate-free column lists: every row gathers the columns of its
+// incident elements into its own stretch of one buffer (sized by the
+// incidence counts), sorts and compacts it; rows are independent (OpenMP).
 void adjacency(const std::vector<int32_t> &el2nod, int64_t nel, int64_t nrow, int nr, const int *rsel,
                int nc, const int *csel, std::vector<int64_t> &ptr, std::vector<int32_t> &cols) {
   // node -> elements
@@ -59,36 +65,25 @@ void adjacency(const std::vector<int32_t> &el2nod, int64_t nel, int64_t nrow, in
     for (int i = 0; i < nr; ++i) eptr[el2nod[10 * e + rsel[i]] + 1]++;
   for (int64_t n = 0; n < nrow; ++n) eptr[n + 1] += eptr[n];
   std::vector<int64_t> fill(eptr.begin(), eptr.end() - 1);
-  std::vector<int64_t> elist(eptr[nrow]);
+  std::vector<int32_t> elist(eptr[nrow]);
   for (int64_t e = 0; e < nel; ++e)
-    for (int i = 0; i < nr; ++i) elist[fill[el2nod[10 * e + rsel[i]]]++] = e;
-  std::vector<std::vector<int32_t>> rows(nrow);
+    for (int i = 0; i < nr; ++i) elist[fill[el2nod[10 * e + rsel[i]]]++] = (int32_t)e;
+  std::vector<int32_t> buf((size_t)eptr[nrow] * nc);
+  std::vector<int64_t> len(nrow);
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int64_t n = 0; n < nrow; ++n) {
-    std::vector<int32_t> &v = rows[n];
+    int32_t *b = buf.data() + eptr[n] * nc, *w = b;
     for (int64_t k = eptr[n]; k < eptr[n + 1]; ++k)
-      for (int j = 0; j < nc; ++j) v.push_back(el2nod[10 * elist[k] + csel[j]]);
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
+      for (int j = 0; j < nc; ++j) *w++ = el2nod[10 * (int64_t)elist[k] + csel[j]];
+    std::sort(b, w);
+    len[n] = std::unique(b, w) - b;
   }
   ptr.assign(nrow + 1, 0);
-  for (int64_t n = 0; n < nrow; ++n) ptr[n + 1] = ptr[n] + (int64_t)rows[n].size();
+  for (int64_t n = 0; n < nrow; ++n) ptr[n + 1] = ptr[n] + len[n];
   cols.resize(ptr[nrow]);
-  for (int64_t n = 0; n < nrow; ++n) std::copy(rows[n].begin(), rows[n].end(), cols.begin() + ptr[n]);
-}
-
-// slot of block (row, col) in a SELL pattern, -1 if absent
-int32_t host_find(const Sell &S, int64_t row, int64_t col) {
-  const int64_t base = S.h_slice_ptr[row >> 6] + (row & 63);
-  int lo = 0, hi = S.h_rowlen[row] - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int32_t cv = S.h_col[base + (int64_t)mid * 64];
-    if (cv == col) return (int32_t)(base + (int64_t)mid * 64);
-    if (cv < col) lo = mid + 1;
-    else hi = mid - 1;
-  }
-  return -1;
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < nrow; ++n)
+    std::copy(buf.begin() + eptr[n] * nc, buf.begin() + eptr[n] * nc + len[n], cols.begin() + ptr[n]);
 }
 
 const int ALL10[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
@@ -200,6 +195,15 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
     M3_REQUIRE(m->nmg >= 2 && m->nmg <= M3_MAX_LEVELS, M3_E_ARG, "nmg must be in 2..8");
     const int nmg = m->nmg;
     c->nmg = nmg;
+    // M3_SETUP_TIMING=1 prints the time of each set-up phase to stderr
+    const bool report = std::getenv("M3_SETUP_TIMING") != nullptr;
+    auto tp = clk::now();
+    auto mark = [&](const char *what) {
+      if (!report) return;
+      const auto now = clk::now();
+      std::fprintf(stderr, "[m3_set_mesh] %-34s %8.3f s\n", what, secs(tp, now));
+      tp = now;
+    };
     c->L.clear();
     c->adv.ready = false;
     c->L.resize(nmg);
@@ -230,6 +234,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       adjacency(lv.h_el2nod, lv.nel, lv.nnod, 10, ALL10, 10, ALL10, ptr, cols);
       lv.K.build(lv.nnod, lv.nnod, ptr, cols, 9);
       if (l < nmg - 1) lv.Kel.alloc((size_t)lv.nel * 465);
+      mark("level connectivity + K pattern");
     }
     LevelDev &f = c->L[0];
     c->nel = f.nel;
@@ -245,27 +250,16 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       c->GT.build(f.nV, f.nnod, ptr, cols, 3);
       adjacency(f.h_el2nod, f.nel, f.nV, 4, V4, 4, V4, ptr, cols);
       c->M.build(f.nV, f.nV, ptr, cols, 1);
+      mark("G, GT, M patterns");
       // slot of every element block in K (10x10), G (10x4), GT (4x10), M (4x4):
       // the assembly scatters without searching the sparse rows
-      std::vector<int32_t> slots((size_t)f.nel * ASM_SLOTS);
-      bool missing = false;
-#pragma omp parallel for schedule(static) reduction(|| : missing)
-      for (int64_t e = 0; e < f.nel; ++e) {
-        const int32_t *n = &f.h_el2nod[10 * e];
-        int32_t *o = &slots[(size_t)e * ASM_SLOTS];
-        for (int a = 0; a < 10; ++a)
-          for (int b = 0; b < 10; ++b) o[10 * a + b] = host_find(f.K, n[a], n[b]);
-        for (int a = 0; a < 10; ++a)
-          for (int v = 0; v < 4; ++v) {
-            o[100 + 4 * a + v] = host_find(c->G, n[a], n[v]);
-            o[140 + 4 * a + v] = host_find(c->GT, n[v], n[a]);
-          }
-        for (int v = 0; v < 4; ++v)
-          for (int w = 0; w < 4; ++w) o[180 + 4 * v + w] = host_find(c->M, n[v], n[w]);
-        for (int k = 0; k < ASM_SLOTS; ++k) missing = missing || o[k] < 0;
-      }
-      M3_REQUIRE(!missing, M3_E_MESH, "element block missing from the sparse pattern");
-      c->asm_slots.upload(slots);
+      M3_REQUIRE(std::max({f.K.nslot, c->G.nslot, c->GT.nslot, c->M.nslot}) < (int64_t)INT32_MAX, M3_E_MESH,
+                 "sparse pattern too large for 32-bit slots");
+      c->asm_slots.alloc((size_t)f.nel * ASM_SLOTS);
+      c->errflag.zero(c->stream);
+      m3::launch_slots(c);
+      raise_flags(check_err(c));
+      mark("element slot map");
       // node -> (element, local node) incidence for the row-owner assembly of K
       std::vector<int64_t> iptr(f.nnod + 1, 0);
       for (int64_t k = 0; k < 10 * f.nel; ++k) ++iptr[f.h_el2nod[k] + 1];
@@ -280,9 +274,11 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       for (int64_t r = 0; r < f.nnod; ++r) maxlen = std::max(maxlen, f.K.h_rowlen[r]);
       c->gather_K = maxlen <= 128 && 10 * f.nel < (int64_t)INT32_MAX;
       f.K.val.zero(c->stream);  // padding slots stay zero
+      mark("node incidence");
     }
     // coordinates and nodal frames (make_rotation_matrix.m:69-110)
     std::vector<double> X(3 * f.nnod), S(3 * f.nnod), fr(9 * f.nnod);
+#pragma omp parallel for schedule(static)
     for (int64_t n = 0; n < f.nnod; ++n) {
       for (int k = 0; k < 3; ++k) {
         X[3 * n + k] = m->GCOORD[3 * n + k];
@@ -327,6 +323,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
           X20[60 * i + 3 * n + k] = g < f.nnod ? m->GCOORD[3 * g + k] : m->GCOORD_cubic[3 * (g - f.nnod) + k];
       }
     c->X20.upload(X20);
+    mark("coordinates, frames, classes");
     // transfers and child interpolation tables
     for (int l = 0; l < nmg - 1; ++l) {
       LevelDev &fl = c->L[l];
@@ -359,34 +356,60 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       fl.R_col.upload(R_col);
       fl.R_val.upload(R_val);
       // Q_e[i][a] = Ic2f(node_e(i), node_parent(a)); every nonzero of the
-      // rows of a child must lie on its parent's nodes (nesting invariant)
-      std::map<std::vector<double>, int32_t> types;
-      std::vector<int32_t> ctype(fl.nel);
-      std::vector<double> qtab;
-      for (int64_t e = 0; e < fl.nel; ++e) {
+      // rows of a child must lie on its parent's nodes (nesting invariant).
+      // Distinct tables are numbered in order of first appearance: a 64-bit
+      // hash per element in parallel, then the tables themselves compared.
+      auto table = [&](int64_t e, double *Q) {
         const int64_t pc = e / 8;
-        std::vector<double> Q(100, 0.0);
+        std::fill(Q, Q + 100, 0.0);
         for (int i = 0; i < 10; ++i) {
           const int node = fl.h_el2nod[10 * e + i];
           for (int64_t k = P_ptr[node]; k < P_ptr[node + 1]; ++k) {
             int a = -1;
             for (int q = 0; q < 10; ++q)
               if (cl.h_el2nod[10 * pc + q] == P_col[k]) a = q;
-            M3_REQUIRE(a >= 0, M3_E_MESH,
-                       "Ic2f row of a fine node references a node outside the parent element");
+            if (a < 0) return false;
             Q[10 * i + a] = P_val[k];
           }
         }
-        auto it = types.find(Q);
+        return true;
+      };
+      std::vector<uint64_t> qhash(fl.nel);
+      bool outside = false;
+#pragma omp parallel for schedule(static) reduction(|| : outside)
+      for (int64_t e = 0; e < fl.nel; ++e) {
+        double Q[100];
+        outside = outside || !table(e, Q);
+        uint64_t h = 1469598103934665603ull;  // FNV-1a over the bytes of the table
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(Q);
+        for (size_t k = 0; k < sizeof(Q); ++k) h = (h ^ b[k]) * 1099511628211ull;
+        qhash[e] = h;
+      }
+      M3_REQUIRE(!outside, M3_E_MESH, "Ic2f row of a fine node references a node outside the parent element");
+      std::unordered_map<uint64_t, int32_t> types;
+      std::vector<int32_t> ctype(fl.nel);
+      std::vector<double> qtab;
+      for (int64_t e = 0; e < fl.nel; ++e) {
+        auto it = types.find(qhash[e]);
         if (it == types.end()) {
-          it = types.emplace(Q, (int32_t)types.size()).first;
-          qtab.insert(qtab.end(), Q.begin(), Q.end());
+          it = types.emplace(qhash[e], (int32_t)types.size()).first;
+          qtab.resize(qtab.size() + 100);
+          table(e, &qtab[qtab.size() - 100]);
         }
         ctype[e] = it->second;
       }
+      bool clash = false;
+#pragma omp parallel for schedule(static) reduction(|| : clash)
+      for (int64_t e = 0; e < fl.nel; ++e) {
+        double Q[100];
+        table(e, Q);
+        clash = clash || std::memcmp(Q, &qtab[100 * (size_t)ctype[e]], sizeof(Q)) != 0;
+      }
+      M3_REQUIRE(!clash, M3_E_MESH, "child interpolation tables with equal hashes differ");
       fl.child_type.upload(ctype);
       fl.Qtab.upload(qtab);
       fl.nqtypes = (int)types.size();
+      mark("transfer + child tables");
     }
     // work space
     int64_t maxpart = 2048;
@@ -405,6 +428,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
     for (auto *b : {&c->pr, &c->prlast, &c->pd, &c->pq, &c->pSq, &c->pP, &c->pMd, &c->LiMM}) b->alloc(f.nV);
     c->visc.alloc(f.nnod);
     c->dens.alloc(f.nnod);
+    mark("work space");
     c->have_mesh = true;
   });
 }

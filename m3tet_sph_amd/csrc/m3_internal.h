@@ -265,6 +265,7 @@ void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double
 void launch_rap(m3_ctx *c, int l);  // level l element matrices -> level l+1
 void launch_fix_diag(m3_ctx *c, int l);
 void launch_gather_K(m3_ctx *c);  // finest K from the element matrices (no atomics)
+void launch_slots(m3_ctx *c);     // per-element slot map of K, G, GT, M (errflag bit 8: missing)
 void launch_diag_inv(m3_ctx *c, int l);
 void launch_lumped_mm(m3_ctx *c);
 

@@ -424,9 +424,10 @@ class ShellMesh:
         Q = child_interp_tables()
         nel_f = len(fine)
         flat = fine.ravel()
-        order = np.argsort(flat, kind="stable")
-        ends = np.searchsorted(flat[order], np.arange(nf), side="right") - 1
-        lastpos = order[ends]                 # flat index of last occurrence
+        lastpos = np.full(nf, -1, dtype=np.int64)      # flat index of last occurrence
+        np.maximum.at(lastpos, flat, np.arange(flat.size, dtype=np.int64))
+        if np.any(lastpos < 0):
+            raise ValueError("a fine node belongs to no element")
         last = lastpos // 10
         pos = lastpos % 10
         parent = last // 8
