@@ -451,6 +451,46 @@ def build_shell(level, nlayer, nmg, r_in=0.546, r_out=1.0,
     """Base icosahedral shell (``level`` x ``nlayer``) refined nmg-1 times;
     with ``device`` the bar numbering of every level runs on that GPU."""
     X4, EL4, pid4 = shell_p1(level, nlayer, r_in, r_out)
+    return build_hierarchy(X4, EL4, pid4, r_in, r_out, nmg, theta_cone, device)
+
+
+def load_mesh_sph(path):
+    """(GCOORD (n,3), EL2NOD (nel,10) 0-based, PointID, r_cmb, r_surf) of a
+    mesh file in the reference's layout (load_mesh_sph.m: struct MESH with
+    GCOORD 3 x nnod, EL2NOD 10 x nel, PointID; negative-volume elements get
+    vertices 3 and 4 swapped; r_cmb / r_surf default to 3471 / 6371)."""
+    from . import matio
+    Mf = matio.loadmat(path)["MESH"]
+    X = np.ascontiguousarray(np.asarray(Mf["GCOORD"], dtype=np.float64).T)
+    EL = np.asarray(Mf["EL2NOD"]).astype(np.int64).T - 1
+    if EL.shape[1] not in (4, 10):
+        raise ValueError("EL2NOD must have 4 or 10 rows")
+    neg = tet_volume6(X, EL[:, :4]) < 0
+    EL[neg, 2], EL[neg, 3] = EL[neg, 3].copy(), EL[neg, 2].copy()
+    pid = np.asarray(Mf["PointID"]).ravel().astype(np.int32)
+    r_cmb = float(np.asarray(Mf.get("r_cmb", 3471.0)).ravel()[0])
+    r_surf = float(np.asarray(Mf.get("r_surf", 6371.0)).ravel()[0])
+    return X, EL, pid, r_cmb, r_surf
+
+
+def build_from_file(path, nmg, theta_cone=THETA_CONE_DEG, scale=None, device=None):
+    """The reference's set-up from one of its mesh files
+    (init_mesh_curved_p.m:60-115): the vertex mesh EL2NOD(1:4,:) promoted to
+    curved P2 (the coarsest level) and refined nmg-1 times.  Coordinates are
+    multiplied by ``scale`` (default 1 / r_surf, so the shell is
+    [r_cmb / r_surf, 1])."""
+    X, EL, pid, r_cmb, r_surf = load_mesh_sph(path)
+    nV = int(EL[:, :4].max()) + 1
+    if scale is None:
+        X4, r_in, r_out = X[:nV] / r_surf, r_cmb / r_surf, 1.0
+    else:
+        X4, r_in, r_out = X[:nV] * float(scale), r_cmb * float(scale), r_surf * float(scale)
+    return build_hierarchy(X4, np.ascontiguousarray(EL[:, :4]), pid[:nV], r_in, r_out, nmg, theta_cone, device)
+
+
+def build_hierarchy(X4, EL4, pid4, r_in, r_out, nmg, theta_cone=THETA_CONE_DEG, device=None):
+    """Coarsest P1 shell -> curved P2 (tetmesh_p1_to_p2_sph) -> nmg-1 nested
+    8-way refinements, each promoted to curved P2 again."""
     SPH4 = cart2sph(X4)
     X, SPH, EL10, pid, classes, inc, _ = p1_to_p2_sph(X4, SPH4, EL4, pid4, r_in, r_out,
                                                       theta_cone, device)
