@@ -368,18 +368,25 @@ __global__ __launch_bounds__(256) void k_gtask(const CholTiles::Item *items, int
   }
 }
 
-__global__ void k_to_tiles(int64_t npos, const int32_t *pos2dof, const double *x, double *w) {
+// right-hand side into tile order (FP64), and the forward / backward
+// accumulators y and x cleared, in one pass
+template <class T>
+__global__ void k_to_tiles(int64_t npos, const int32_t *pos2dof, const T *rhs, double *w, double *y, double *x) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= npos) return;
   const int d = pos2dof[i];
-  w[i] = d >= 0 ? x[d] : 0.0;
+  w[i] = d >= 0 ? (double)rhs[d] : 0.0;
+  y[i] = 0.0;
+  x[i] = 0.0;
 }
 
-__global__ void k_from_tiles(int64_t npos, const int32_t *pos2dof, const double *w, double *x) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= npos) return;
-  const int d = pos2dof[i];
-  if (d >= 0) x[d] = w[i];
+// solution back to dof order; fixed dofs (no tile position) get 0
+template <class T>
+__global__ void k_from_tiles(int64_t n, const int32_t *dof2pos, const double *x, T *out) {
+  const int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  const int p = dof2pos[d];
+  out[d] = p >= 0 ? (T)x[p] : (T)0;
 }
 
 __global__ void k_zero(int64_t n, double *x) {
@@ -678,6 +685,7 @@ void chol_analyse(m3_ctx *c) {
   std::vector<int32_t> diagt(T);
   for (int J = 0; J < T; ++J) diagt[J] = ch.h_colptr[J];
   up(ch.pos2dof, pos2dof);
+  up(ch.dof2pos, inv);
   up(ch.lev_cols, lev_cols);
   up(ch.upd_tile, upd_tile);
   up(ch.upd_pair_ptr, pair_beg);
@@ -748,20 +756,22 @@ static void run_phase(hipStream_t s, CholTiles::Phase &p, int h, const double *m
     hipLaunchKernelGGL(k_gtask, dim3(b - a), dim3(256), 0, s, p.items.p, a, mats, in0, in1, out0, out1);
 }
 
-void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) {
+template <class T>
+static void chol_solve_t(m3_ctx *c, const T *rhs, T *x) {
   CholTiles &ch = c->chol;
   LevelDev &cl = c->L[c->nmg - 1];
   hipStream_t s = c->stream;
   const int64_t npos = (int64_t)ch.T * TS;
-  hipLaunchKernelGGL(k_to_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.w.p);
-  hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.y.p);
-  hipLaunchKernelGGL(k_zero, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.x.p);
+  hipLaunchKernelGGL(k_to_tiles<T>, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, rhs, ch.w.p, ch.y.p,
+                     ch.x.p);
   for (int h = 0; h < ch.H; ++h) run_phase(s, ch.fwd, h, ch.X.p, ch.w.p, ch.w.p, ch.y.p, ch.w.p);
   for (int k = 0; k < ch.H; ++k) run_phase(s, ch.bwd, k, ch.XT.p, ch.y.p, ch.x.p, ch.x.p, ch.x.p);
   const int64_t n3 = 3 * cl.nnod;
-  hipLaunchKernelGGL(k_zero, dim3(nblk(n3)), dim3(256), 0, s, n3, x);
-  hipLaunchKernelGGL(k_from_tiles, dim3(nblk(npos)), dim3(256), 0, s, npos, ch.pos2dof.p, ch.x.p, x);
+  hipLaunchKernelGGL(k_from_tiles<T>, dim3(nblk(n3)), dim3(256), 0, s, n3, ch.dof2pos.p, ch.x.p, x);
   HIPCHECK(hipGetLastError());
 }
+
+void chol_solve_dev(m3_ctx *c, const double *rhs, double *x) { chol_solve_t<double>(c, rhs, x); }
+void chol_solve_dev32(m3_ctx *c, const float *rhs, float *x) { chol_solve_t<float>(c, rhs, x); }
 
 }  // namespace m3

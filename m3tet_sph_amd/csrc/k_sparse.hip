@@ -259,10 +259,15 @@ __global__ __launch_bounds__(256) void k_restrict(int64_t nc, const int64_t *ptr
   }
 }
 
+// V-cycle entry on the finest level: r0 = (VT) r and the first smoother
+// iterate d = w1 Dinv r0
 template <class VT>
-__global__ void k_scale(int64_t n, VT w, const VT *dinv, const VT *r, VT *d) {
+__global__ void k_entry(int64_t n, const double *r, const VT *dinv, VT w, VT *r0, VT *d) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) d[i] = w * dinv[i] * r[i];
+  if (i >= n) return;
+  const VT ri = (VT)r[i];
+  r0[i] = ri;
+  d[i] = w * dinv[i] * ri;
 }
 
 template <class A, class B>
@@ -378,32 +383,35 @@ __global__ void k_cg_beta(const double *part, int nb, double *scal, int first) {
   }
 }
 
-__global__ void k_cg_alpha(const double *part, int nb, double *scal) {
-  double s = 0;
-  for (int i = threadIdx.x; i < nb; i += RB) s += part[i];
-  __shared__ double sh[RB];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = RB / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    scal[4] = sh[0];
-    scal[3] = scal[0] / sh[0];  // alpha = rd / qKq (:173)
-  }
-}
-
 // q = d + beta q
 __global__ void k_cg_dir(int64_t n, const double *d, double *q, const double *scal) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) q[i] = d[i] + scal[2] * q[i];
 }
 
-// x += alpha q; rlast = r; r -= alpha Kq; partial |r|^2
-__global__ void k_cg_update(int64_t n, double *x, const double *q, double *r, double *rlast,
-                            const double *Kq, const double *scal, double *part) {
-  const double alpha = scal[3];
+// alpha = rd / qKq from the SpMV's partial sums (every block reduces them in
+// the same order, so all blocks hold the same alpha; block 0 records it),
+// then x += alpha q; rlast = r; r -= alpha Kq; partial |r|^2
+__global__ void k_cg_alpha_update(int64_t n, double *x, const double *q, double *r, double *rlast,
+                                  const double *Kq, const double *qpart, int nq, double *scal, double *part) {
+  __shared__ double sh[RB];
+  {
+    double s = 0;
+    for (int i = threadIdx.x; i < nq; i += RB) s += qpart[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = RB / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+  }
+  const double qKq = sh[0];
+  const double alpha = scal[0] / qKq;  // alpha = rd / qKq (:173)
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[4] = qKq;
+    scal[3] = alpha;
+  }
   double s = 0;
   for (int64_t i = blockIdx.x * (int64_t)RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
     x[i] += alpha * q[i];
@@ -413,7 +421,6 @@ __global__ void k_cg_update(int64_t n, double *x, const double *q, double *r, do
     r[i] = rn;
     s += rn * rn;
   }
-  __shared__ double sh[RB];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int st = RB / 2; st > 0; st >>= 1) {
@@ -600,9 +607,7 @@ static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
     LevelDev &lv = c->L[l];
     VecSet<VT> v = vs(lv);
     const double *wd = l == 0 ? WDOWN1 : WDOWN;
-    const int64_t n3 = 3 * lv.nnod;
-    if (l == 0)
-      hipLaunchKernelGGL((k_scale<VT>), dim3(nblk(n3)), dim3(256), 0, s, n3, (VT)wd[0], v.dinv, v.r0, v.d);
+    // level 0: r0 and d = w1 Dinv r0 were written by k_entry
     // d (= w1 Dinv r0) is in v.d; rr = r0 is kept (v.r0 not overwritten)
     const int pk = std::is_same<VT, double>::value ? 0 : 1;
     pick_K(l, [&](auto K) {
@@ -628,13 +633,8 @@ static void vcycle_impl(m3_ctx *c, Pick pick_K, VecSet<VT> (*vs)(LevelDev &)) {
   {
     LevelDev &cl = c->L[nm - 1];
     const int64_t n3 = 3 * cl.nnod;
-    if constexpr (std::is_same<VT, double>::value) {
-      chol_solve_dev(c, cl.r0.p, cl.d.p);
-    } else {
-      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n3)), dim3(256), 0, s, n3, cl.r0f.p, cl.r0.p);
-      chol_solve_dev(c, cl.r0.p, cl.d.p);
-      hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, cl.d.p, cl.df.p);
-    }
+    if constexpr (std::is_same<VT, double>::value) chol_solve_dev(c, cl.r0.p, cl.d.p);
+    else chol_solve_dev32(c, cl.r0f.p, cl.df.p);  // FP64 inside, FP32 at the boundary
   }
   for (int l = nm - 2; l >= 0; --l) {
     LevelDev &lv = c->L[l];
@@ -664,11 +664,13 @@ void vcycle(m3_ctx *c, const double *r, double *d, int precision) {
   LevelDev &f = c->L[0];
   const int64_t n3 = 3 * f.nnod;
   if (precision == M3_PREC_FP64) {
-    hipLaunchKernelGGL(k_copy, dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.r0.p);
+    hipLaunchKernelGGL((k_entry<double>), dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.dinv.p, WDOWN1[0], f.r0.p,
+                       f.d.p);
     vcycle_impl<double>(c, [&](int l, auto fn) { fn(c->L[l].K.view()); }, vecs64);
     hipLaunchKernelGGL(k_copy, dim3(nblk(n3)), dim3(256), 0, s, n3, f.d.p, d);
   } else {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.r0f.p);
+    hipLaunchKernelGGL((k_entry<float>), dim3(nblk(n3)), dim3(256), 0, s, n3, r, f.dinv32.p, (float)WDOWN1[0],
+                       f.r0f.p, f.df.p);
     vcycle_impl<float>(
         c,
         [&](int l, auto fn) {
@@ -804,9 +806,10 @@ int mgpcg_run(m3_ctx *c, double *x, const double *b, double tol, bool is_abs, in
     launch_smooth<double, double, 9>(s, f.K.view(), f.nnod, q, (const double *)nullptr,
                                      (const double *)nullptr, 0.0, Kq, (double *)nullptr, (double *)nullptr,
                                      c->partial.p);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(RB), 0, s, c->partial.p, (int)gK, c->scal.p);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(RB), 0, s, n3, x, q, r, rl, Kq, c->scal.p, c->partial.p);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, c->scal.p + 5);
+    double *upart = c->partial.p + c->partial.n / 2;  // the SpMV's partials stay readable
+    hipLaunchKernelGGL(k_cg_alpha_update, dim3(nb), dim3(RB), 0, s, n3, x, q, r, rl, Kq, c->partial.p, (int)gK,
+                       c->scal.p, upart);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, upart, nb, c->scal.p + 5);
     HIPCHECK(hipMemcpyAsync(c->h_scal + 5, c->scal.p + 5, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     rrms = std::sqrt(c->h_scal[5] / nfree);
@@ -842,9 +845,10 @@ int pcg_diag_run(m3_ctx *c, SellView A, int64_t n, const double *Adi, double *x,
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(RB), 0, s, c->partial.p, nb, sc, it == 1 ? 1 : 0);
     hipLaunchKernelGGL(k_cg_dir, dim3(nblk(n)), dim3(256), 0, s, n, d, q, sc);
     hipLaunchKernelGGL(k_spmv1, dim3(gA), dim3(RB), 0, s, A, n, q, Aq, c->partial.p);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(RB), 0, s, c->partial.p, (int)gA, sc);
-    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(RB), 0, s, n, x, q, r, rl, Aq, sc, c->partial.p);
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, c->partial.p, nb, sc + 5);
+    double *upart = c->partial.p + c->partial.n / 2;
+    hipLaunchKernelGGL(k_cg_alpha_update, dim3(nb), dim3(RB), 0, s, n, x, q, r, rl, Aq, c->partial.p, (int)gA, sc,
+                       upart);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(RB), 0, s, upart, nb, sc + 5);
     HIPCHECK(hipMemcpyAsync(c->h_scal + 21, sc + 5, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     rr = c->h_scal[21];

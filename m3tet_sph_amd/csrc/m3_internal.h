@@ -112,6 +112,7 @@ struct CholTiles {
   int64_t ntile = 0;             // stored lower tiles of L (incl. diagonal)
   std::vector<int32_t> h_colptr, h_rowtile;   // tile CSC (row tiles sorted)
   DBuf<int32_t> pos2dof;         // T*64: coarse dof (0-based) or -1 (padding)
+  DBuf<int32_t> dof2pos;         // 3*nnod: tile position or -1 (fixed dof)
   DBuf<double> tiles, tilesT;    // L tiles (column-major) and transposed copies
   DBuf<double> dinv, dinvT;      // inverses of the diagonal tiles of L
   DBuf<int64_t> gsrc, gdst;      // gather: operator value index -> tile value index
@@ -281,6 +282,7 @@ void restrict_(hipStream_t s, LevelDev &fine, LevelDev &coarse, const double *rf
 void vcycle(m3_ctx *c, const double *r, double *d, int precision);
 void chol_factor(m3_ctx *c);
 void chol_solve_dev(m3_ctx *c, const double *rhs, double *x);  // device vectors, coarse level
+void chol_solve_dev32(m3_ctx *c, const float *rhs, float *x);   // the same, FP32 in and out
 
 // vector helpers
 double dot(m3_ctx *c, const double *a, const double *b, int64_t n);
