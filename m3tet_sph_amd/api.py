@@ -224,6 +224,59 @@ def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=1
     return o
 
 
+def mesh_arrays(m):
+    """The arrays of m3_mesh_desc for a ShellMesh, in the reference's layout
+    (1-based indices, EL2NOD as nel rows of 10, Ic2f as node-level CSR)."""
+    out = {}
+    for l, L in enumerate(m.levels):
+        out[f"EL2NOD_{l}"] = np.ascontiguousarray(L.EL2NOD + 1, dtype=np.uint32)
+    out["GCOORD"] = np.ascontiguousarray(m.GCOORD, dtype=np.float64)
+    out["GCOORD_SPH"] = np.ascontiguousarray(m.GCOORD_SPH, dtype=np.float64)
+    names = ("out_cone_no_cross_2pi", "out_cone_cross_2pi", "in_cone_no_iso", "in_cone_iso")
+    for k, nm in enumerate(names):
+        out[f"class_{k}"] = np.ascontiguousarray(m.classes[nm] + 1, dtype=np.int32)
+    iso = m.classes["in_cone_iso"]
+    n_iso = len(iso)
+    E20 = np.zeros((max(n_iso, 1), 20), dtype=np.uint32)
+    Xc = np.zeros((max(n_iso * 16, 1), 3))
+    if n_iso:
+        E20[:, :4] = m.levels[0].EL2NOD[iso, :4] + 1
+        E20[:, 4:] = m.nnod + 1 + np.arange(n_iso * 16).reshape(n_iso, 16)
+        Xc = np.ascontiguousarray(m.X20_iso[:, 4:, :].reshape(-1, 3))
+    out["EL2NOD_cubic"], out["GCOORD_cubic"], out["ncubic"] = E20, Xc, n_iso * 16
+    for l in range(m.nmg - 1):
+        ip, ci, w = m.transfer(l)
+        out[f"Ic2f_ptr_{l}"] = np.ascontiguousarray(ip, dtype=np.int64)
+        out[f"Ic2f_col_{l}"] = np.ascontiguousarray(ci, dtype=np.int32)
+        out[f"Ic2f_val_{l}"] = np.ascontiguousarray(w, dtype=np.float64)
+    return out
+
+
+def export_case(path, m, inputs):
+    """Write a solve case for the C++ harness ``m3bench``: the m3_mesh_desc
+    arrays, UBC.ifix / UBC.vfix (1-based), nodal Visc / Dens as .npy files and
+    the sizes and PHYSICS / NUMSCALE scalars in case.txt ("key value" lines)."""
+    os.makedirs(path, exist_ok=True)
+    arrs = mesh_arrays(m)
+    for k, v in arrs.items():
+        if isinstance(v, np.ndarray):
+            np.save(os.path.join(path, k + ".npy"), v)
+    np.save(os.path.join(path, "ifix.npy"), np.ascontiguousarray(np.asarray(inputs["ifix"]) + 1, dtype=np.int64))
+    np.save(os.path.join(path, "vfix.npy"), np.ascontiguousarray(inputs["vfix"], dtype=np.float64))
+    np.save(os.path.join(path, "Visc.npy"), np.ascontiguousarray(inputs["Visc"], dtype=np.float64))
+    np.save(os.path.join(path, "Dens.npy"), np.ascontiguousarray(inputs["Dens"], dtype=np.float64))
+    ph = inputs["phys"]
+    lines = [f"nmg {m.nmg}", f"ncubic {arrs['ncubic']}"]
+    for l, L in enumerate(m.levels):
+        lines += [f"nel_{l} {len(L.EL2NOD)}", f"nnod_{l} {L.nnod}", f"nVnod_{l} {L.nVnod}"]
+    lines += [f"g {ph['g']!r}", f"Bscale {ph['Bscale']!r}", f"Dens0 {ph['Dens0']!r}"]
+    for k in ("minVisc", "maxVisc"):
+        if ph.get(k) is not None:
+            lines.append(f"{k} {ph[k]!r}")
+    with open(os.path.join(path, "case.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 class StokesContext:
     """Device state of one mesh (patterns, operators, hierarchy, factor)."""
 
@@ -256,41 +309,22 @@ class StokesContext:
             raise M3Error(rc, LIB.m3_last_error(self._h).decode())
 
     def _set_mesh(self, m):
+        arrs = mesh_arrays(m)
         d = MeshDesc()
         d.nmg = m.nmg
-        keep = []
         for l, L in enumerate(m.levels):
             d.nel[l], d.nnod[l], d.nVnod[l] = len(L.EL2NOD), L.nnod, L.nVnod
-            el = np.ascontiguousarray(L.EL2NOD + 1, dtype=np.uint32)
-            keep.append(el)
-            d.EL2NOD[l] = _p(el, U32)
-        X = np.ascontiguousarray(m.GCOORD, dtype=np.float64)
-        S = np.ascontiguousarray(m.GCOORD_SPH, dtype=np.float64)
-        keep += [X, S]
-        d.GCOORD, d.GCOORD_SPH = _p(X), _p(S)
-        names = ("out_cone_no_cross_2pi", "out_cone_cross_2pi", "in_cone_no_iso", "in_cone_iso")
-        for k, nm in enumerate(names):
-            a = np.ascontiguousarray(m.classes[nm] + 1, dtype=np.int32)
-            keep.append(a)
-            d.els_class[k] = _p(a, I32)
-            d.nels_class[k] = len(a)
-        iso = m.classes["in_cone_iso"]
-        n_iso = len(iso)
-        E20 = np.zeros((max(n_iso, 1), 20), dtype=np.uint32)
-        Xc = np.zeros((max(n_iso * 16, 1), 3))
-        if n_iso:
-            E20[:, :4] = m.levels[0].EL2NOD[iso, :4] + 1
-            E20[:, 4:] = m.nnod + 1 + np.arange(n_iso * 16).reshape(n_iso, 16)
-            Xc = np.ascontiguousarray(m.X20_iso[:, 4:, :].reshape(-1, 3))
-        keep += [E20, Xc]
-        d.EL2NOD_cubic, d.GCOORD_cubic, d.ncubic = _p(E20, U32), _p(Xc), n_iso * 16
+            d.EL2NOD[l] = _p(arrs[f"EL2NOD_{l}"], U32)
+        d.GCOORD, d.GCOORD_SPH = _p(arrs["GCOORD"]), _p(arrs["GCOORD_SPH"])
+        for k in range(4):
+            d.els_class[k] = _p(arrs[f"class_{k}"], I32)
+            d.nels_class[k] = len(arrs[f"class_{k}"])
+        d.EL2NOD_cubic, d.GCOORD_cubic = _p(arrs["EL2NOD_cubic"], U32), _p(arrs["GCOORD_cubic"])
+        d.ncubic = int(arrs["ncubic"])
         for l in range(m.nmg - 1):
-            ip, ci, w = m.transfer(l)
-            ip = np.ascontiguousarray(ip, dtype=np.int64)
-            ci = np.ascontiguousarray(ci, dtype=np.int32)
-            w = np.ascontiguousarray(w, dtype=np.float64)
-            keep += [ip, ci, w]
-            d.Ic2f_ptr[l], d.Ic2f_col[l], d.Ic2f_val[l] = _p(ip, I64), _p(ci, I32), _p(w)
+            d.Ic2f_ptr[l] = _p(arrs[f"Ic2f_ptr_{l}"], I64)
+            d.Ic2f_col[l] = _p(arrs[f"Ic2f_col_{l}"], I32)
+            d.Ic2f_val[l] = _p(arrs[f"Ic2f_val_{l}"])
         self._check(LIB.m3_set_mesh(self._h, C.byref(d)))
 
     # ---- operations ------------------------------------------------------------

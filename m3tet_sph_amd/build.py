@@ -9,6 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libm3stokes.so")
+BENCH = os.path.join(HERE, "m3bench")
 SOURCES = ["m3_api.hip", "k_assembly.hip", "k_sparse.hip", "k_chol.hip", "k_advect.hip", "k_gnn.hip", "k_material.hip",
            "k_diffusion.hip", "k_mesh.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -50,6 +51,16 @@ def build(verbose=True, jobs=4):
         os.remove(o)
     if verbose:
         sys.stderr.write(f"[build] wrote {LIB}\n")
+    # the standalone C++ harness of the C ABI, next to the library it links
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I", inc, os.path.join(CSRC, "m3bench.cpp"),
+           "-o", BENCH + ".tmp", "-L", HERE, "-lm3stokes", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("m3bench build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(BENCH + ".tmp", BENCH)
+    if verbose:
+        sys.stderr.write(f"[build] wrote {BENCH}\n")
     return LIB
 
 

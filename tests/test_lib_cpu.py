@@ -77,3 +77,19 @@ def test_bcs(c0, c0_inputs):
     surf = np.nonzero(pid == 306)[0]
     assert np.all(np.isin(3 * surf + 2, ifix))
     assert np.abs(vfix).max() > 0.1            # plate velocities of unit rate poles
+
+
+def test_cpp_harness_reads_a_case_and_reports_a_missing_device(tmp_path, c0, c0_inputs):
+    """m3bench (the C++ harness of the C ABI) loads an exported case; without
+    a GPU it stops at m3_create with the library's message."""
+    import subprocess
+    from m3tet_sph_amd import api
+    exe = os.path.join(os.path.dirname(api.LIB_PATH), "m3bench")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "usage: m3bench" in r.stderr
+    api.export_case(str(tmp_path), c0, c0_inputs)
+    r = subprocess.run([exe, str(tmp_path), "--steps", "1"], capture_output=True, text=True)
+    if r.returncode == 0:
+        pytest.skip("a GPU is visible")
+    assert "m3_create failed" in r.stderr
