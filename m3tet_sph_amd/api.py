@@ -537,6 +537,56 @@ def mgpcg_solver_p(ctx, b, x, OPTS):
     return x, it, time.perf_counter() - t0, hist
 
 
+def intermesh_transfer_matrix(MESH, l=0, ndof=3):
+    """Ic2f = intermesh_transfer_matrix(GCOORD,EL2NOD,GCOORD_c,EL2NOD_c,ndof)
+    (intermesh_transfer_matrix.m:1-79) between level l (fine) and level l+1
+    (coarse) of MESH: the P2 interpolation of the parent element at every
+    fine node, one row per fine dof (the ndof components share the weights,
+    :50-55), as a SciPy CSR matrix."""
+    ip, ci, w = MESH.transfer(l)
+    nf, nc = MESH.levels[l].nnod, MESH.levels[l + 1].nnod
+    rows = np.repeat(np.arange(nf), np.diff(ip))
+    R = np.concatenate([ndof * rows + k for k in range(ndof)])
+    Cc = np.concatenate([ndof * ci + k for k in range(ndof)])
+    return sp.csr_matrix((np.tile(w, ndof), (R, Cc)), shape=(ndof * nf, ndof * nc))
+
+
+def restrict_stiffmat(MESH, K1=None, ifree=None, fid=None):
+    """[KK,Ic2f_U] = restrict_stiffmat(MESH,K1,ifree,fid) (restrict_stiffmat.m:1):
+    the Galerkin operators of every multigrid level, formed on the device
+    from the K1 assembled in MESH's context (K1 and ifree are held there;
+    the arguments are accepted for the reference's signature), returned as
+    SciPy BSR matrices (full dof numbering, identity at fixed dofs) together
+    with the dof-level transfer matrices."""
+    ctx = _ctx(MESH)
+    ctx.restrict_stiffmat()
+    KK = [get_operator(ctx, l) for l in range(MESH.nmg)]
+    Ic2f_U = [intermesh_transfer_matrix(MESH, l) for l in range(MESH.nmg - 1)]
+    return KK, Ic2f_U
+
+
+def cholesky_factorization(MESH):
+    """[L,perm] = cholesky_factorization(A) (cholesky_factorization.m:26-36)
+    for the coarsest operator of MESH's context: the tile factor stays on the
+    device; the returned handle is what ``cholesky_solve`` takes."""
+    ctx = _ctx(MESH)
+    ctx.cholesky_factorization()
+    return ctx
+
+
+def cholesky_solve(rhs, L):
+    """x = cholesky_solve(rhs,L,perm) (cholesky_solve.m:27-43) with the handle
+    of ``cholesky_factorization``; rhs and x in the coarse dof numbering
+    (fixed dofs give 0)."""
+    return L.cholesky_solve(rhs)
+
+
+def flow3d_patera_mg_sph_p(VAR, MESH, COMM, UBC, SETTINGS, PHYSICS, NUMSCALE, dt=None, **opts):
+    """[VAR,Uinfo] = flow3d_patera_mg_sph_p(...) (flow3d_patera_mg_sph_p.m:1): the
+    multigrid Stokes solve that flow3d_patera_sph dispatches to."""
+    return flow3d_patera_sph(VAR, MESH, COMM, UBC, SETTINGS, PHYSICS, NUMSCALE, dt, **opts)
+
+
 def flow3d_patera_sph(VAR, MESH, COMM, UBC, SETTINGS, PHYSICS, NUMSCALE, dt=None, **opts):
     """[VAR,Uinfo] = flow3d_patera_sph(...) (flow3d_patera_sph.m:1-9): the MG
     branch (nmg > 1) of flow3d_patera_mg_sph_p; returns Ux,Uy,Uz (Cartesian),

@@ -273,3 +273,29 @@ def test_c1_mixed_precision_reaches_tolerance(c1ctx):
     assert rU / fU < 1e-6
     assert rP / i64["norm_rP0"] < 1e-6
     assert eu < 1e-4 and ep < 1e-4
+
+
+def test_reference_named_restriction_and_cholesky(c0, c0_inputs, ora):
+    """restrict_stiffmat / intermesh_transfer_matrix / cholesky_factorization /
+    cholesky_solve with the reference's argument meaning, on a context bound
+    to the mesh, against the oracle's Galerkin operator and dense solve."""
+    from m3tet_sph_amd import api
+    i = c0_inputs
+    ctx = api._ctx(c0)
+    try:
+        ctx.set_bc(i["ifix"], i["vfix"])
+        ctx.assemble(i["Visc"], i["Dens"], i["phys"])
+        KK, Ic2f_U = api.restrict_stiffmat(c0, None, ora["St"]["ifree"], None)
+        ifr = ora["St"]["ifree"]
+        n3c = 3 * c0.levels[1].nnod
+        fr = ifr[ifr < n3c]
+        assert relerr(KK[1].tocsr()[fr][:, fr].toarray(), ora["KKl"][1].toarray()) < 1e-12
+        assert relerr(Ic2f_U[0][ifr][:, fr].toarray(), ora["IcU"][0].toarray()) == 0.0
+        L = api.cholesky_factorization(c0)
+        b = np.zeros(n3c)
+        b[fr] = np.random.default_rng(9).standard_normal(len(fr))
+        x = api.cholesky_solve(b, L)
+        assert relerr(x[fr], ora["chol"].solve(b[fr])) < 1e-11
+    finally:
+        ctx.close()
+        del c0._m3ctx

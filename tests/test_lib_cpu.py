@@ -93,3 +93,13 @@ def test_cpp_harness_reads_a_case_and_reports_a_missing_device(tmp_path, c0, c0_
     if r.returncode == 0:
         pytest.skip("a GPU is visible")
     assert "m3_create failed" in r.stderr
+
+
+def test_intermesh_transfer_matrix_matches_the_oracle(c0):
+    from m3tet_sph_amd import api
+    from oracle import stokes as S
+    for l in range(c0.nmg - 1):
+        ip, ci, w = c0.transfer(l)
+        ref = S.ic2f_dof(ip, ci, w, c0.levels[l + 1].nnod)
+        got = api.intermesh_transfer_matrix(c0, l)
+        assert got.shape == ref.shape and abs(got - ref).max() == 0.0
