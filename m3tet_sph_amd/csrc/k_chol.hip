@@ -42,9 +42,44 @@ struct NDState {
   std::vector<std::vector<int32_t>> blocks;  // node lists in elimination order
 };
 
+// Order the nodes of a block along a 3-D Morton (Z-order) curve of their
+// coordinates: the nodes of a separator that border one subtree then sit in
+// few 64-row tiles, which keeps the tile-level fill of the factor small.
+void spatial_order(const NDState &st, std::vector<int32_t> &nodes) {
+  if (nodes.size() < 2) return;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int n : nodes)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], (*st.xyz)[3 * n + k]);
+      hi[k] = std::max(hi[k], (*st.xyz)[3 * n + k]);
+    }
+  auto spread = [](uint64_t v) {  // 21 bits -> every third bit of 63
+    v &= 0x1fffffull;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int32_t>> key(nodes.size());
+  for (size_t i = 0; i < nodes.size(); ++i) {
+    uint64_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      const double w = hi[k] - lo[k];
+      const double t = w > 0 ? ((*st.xyz)[3 * nodes[i] + k] - lo[k]) / w : 0.0;
+      code |= spread((uint64_t)(t * 2097151.0)) << k;
+    }
+    key[i] = {code, nodes[i]};
+  }
+  std::sort(key.begin(), key.end());
+  for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = key[i].second;
+}
+
 void nd_rec(NDState &st, std::vector<int32_t> nodes) {
   if (nodes.empty()) return;
   if (nodes.size() <= (size_t)LEAF) {
+    spatial_order(st, nodes);
     st.blocks.push_back(nodes);
     return;
   }
@@ -87,6 +122,7 @@ void nd_rec(NDState &st, std::vector<int32_t> nodes) {
   for (int n : B)
     if (st.stamp[n] != sS) B2.push_back(n);
   std::vector<int32_t> Sv = S;
+  spatial_order(st, Sv);
   nd_rec(st, A2);
   nd_rec(st, B2);
   if (!Sv.empty()) st.blocks.push_back(Sv);
