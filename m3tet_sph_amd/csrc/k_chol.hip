@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <array>
+#include <functional>
+#include <unordered_map>
 #include <map>
 #include <numeric>
 
@@ -76,6 +78,105 @@ void spatial_order(const NDState &st, std::vector<int32_t> &nodes) {
   for (size_t i = 0; i < nodes.size(); ++i) nodes[i] = key[i].second;
 }
 
+// Minimum vertex cover of the bipartite graph between bA and bB (edges of
+// the adjacency between the two sides; bB nodes carry stamp sB): Hopcroft-
+// Karp maximum matching, then (bA \ Z) + (bB & Z) with Z the nodes reachable
+// from unmatched bA nodes by alternating paths.  Every A-B edge has an end in
+// the cover, so it separates A and B.
+std::vector<int32_t> min_cover(NDState &st, const std::vector<int32_t> &bA, const std::vector<int32_t> &bB, int sB) {
+  const int na = (int)bA.size(), nb = (int)bB.size();
+  if (na == 0 || nb == 0) return {};
+  std::unordered_map<int32_t, int> idB;
+  idB.reserve(2 * nb);
+  for (int j = 0; j < nb; ++j) idB[bB[j]] = j;
+  std::vector<int64_t> gp(na + 1, 0);
+  std::vector<int> gadj;
+  for (int i = 0; i < na; ++i) {
+    const int n = bA[i];
+    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k) {
+      const int m = (*st.adj)[k];
+      if (st.stamp[m] != sB) continue;
+      auto it = idB.find(m);
+      if (it != idB.end()) gadj.push_back(it->second);
+    }
+    gp[i + 1] = (int64_t)gadj.size();
+  }
+  // Hopcroft-Karp
+  std::vector<int> mA(na, -1), mB(nb, -1), dist(na);
+  const int INF = 1 << 30;
+  auto bfs = [&]() {
+    std::vector<int> q;
+    q.reserve(na);
+    bool found = false;
+    for (int i = 0; i < na; ++i) {
+      if (mA[i] < 0) {
+        dist[i] = 0;
+        q.push_back(i);
+      } else {
+        dist[i] = INF;
+      }
+    }
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int i = q[h];
+      for (int64_t k = gp[i]; k < gp[i + 1]; ++k) {
+        const int j = gadj[k], i2 = mB[j];
+        if (i2 < 0) found = true;
+        else if (dist[i2] == INF) {
+          dist[i2] = dist[i] + 1;
+          q.push_back(i2);
+        }
+      }
+    }
+    return found;
+  };
+  std::vector<int64_t> itk(na);
+  std::function<bool(int)> dfs = [&](int i) -> bool {
+    for (int64_t &k = itk[i]; k < gp[i + 1]; ++k) {
+      const int j = gadj[k], i2 = mB[j];
+      if (i2 < 0 || (dist[i2] == dist[i] + 1 && dfs(i2))) {
+        mA[i] = j;
+        mB[j] = i;
+        ++k;
+        return true;
+      }
+    }
+    dist[i] = INF;
+    return false;
+  };
+  while (bfs()) {
+    for (int i = 0; i < na; ++i) itk[i] = gp[i];
+    for (int i = 0; i < na; ++i)
+      if (mA[i] < 0) dfs(i);
+  }
+  // Koenig: alternating reachability from unmatched bA nodes
+  std::vector<char> zA(na, 0), zB(nb, 0);
+  std::vector<int> q;
+  for (int i = 0; i < na; ++i)
+    if (mA[i] < 0) {
+      zA[i] = 1;
+      q.push_back(i);
+    }
+  for (size_t h = 0; h < q.size(); ++h) {
+    const int i = q[h];
+    for (int64_t k = gp[i]; k < gp[i + 1]; ++k) {
+      const int j = gadj[k];
+      if (zB[j] || mA[i] == j) continue;  // non-matching edges A -> B
+      zB[j] = 1;
+      const int i2 = mB[j];
+      if (i2 >= 0 && !zA[i2]) {
+        zA[i2] = 1;
+        q.push_back(i2);
+      }
+    }
+  }
+  std::vector<int32_t> cover;
+  for (int i = 0; i < na; ++i)
+    if (!zA[i]) cover.push_back(bA[i]);
+  for (int j = 0; j < nb; ++j)
+    if (zB[j]) cover.push_back(bB[j]);
+  return cover;
+}
+
 void nd_rec(NDState &st, std::vector<int32_t> nodes) {
   if (nodes.empty()) return;
   if (nodes.size() <= (size_t)LEAF) {
@@ -112,8 +213,10 @@ void nd_rec(NDState &st, std::vector<int32_t> nodes) {
     for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
       if (st.stamp[(*st.adj)[k]] == sA) { bB.push_back(n); break; }
   }
-  const bool sepA = bA.size() <= bB.size();
-  const std::vector<int32_t> &S = sepA ? bA : bB;
+  // smallest vertex separator between the two boundary layers: a minimum
+  // vertex cover of the bipartite graph of the A-B edges (Koenig: maximum
+  // matching, then the alternating-path closure of the unmatched bA nodes)
+  const std::vector<int32_t> S = min_cover(st, bA, bB, sB);
   const int sS = ++st.cur;
   for (int n : S) st.stamp[n] = sS;
   std::vector<int32_t> A2, B2;
