@@ -177,6 +177,45 @@ std::vector<int32_t> min_cover(NDState &st, const std::vector<int32_t> &bA, cons
   return cover;
 }
 
+// Median split of `nodes` across coordinate axis `ax` into A | B and the
+// minimum vertex separator S between them (S is taken out of A and B).
+void split(NDState &st, std::vector<int32_t> nodes, int ax, std::vector<int32_t> &A2, std::vector<int32_t> &B2,
+           std::vector<int32_t> &S) {
+  const size_t mid = nodes.size() / 2;
+  std::nth_element(nodes.begin(), nodes.begin() + mid, nodes.end(), [&](int a, int b) {
+    const double xa = (*st.xyz)[3 * a + ax], xb = (*st.xyz)[3 * b + ax];
+    return xa < xb || (xa == xb && a < b);
+  });
+  std::vector<int32_t> A(nodes.begin(), nodes.begin() + mid), B(nodes.begin() + mid, nodes.end());
+  const int sA = ++st.cur;
+  for (int n : A) st.stamp[n] = sA;
+  const int sB = ++st.cur;
+  for (int n : B) st.stamp[n] = sB;
+  std::vector<int32_t> bA, bB;  // boundary layers
+  for (int n : A) {
+    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
+      if (st.stamp[(*st.adj)[k]] == sB) { bA.push_back(n); break; }
+  }
+  for (int n : B) {
+    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
+      if (st.stamp[(*st.adj)[k]] == sA) { bB.push_back(n); break; }
+  }
+  // smallest vertex separator between the two boundary layers: a minimum
+  // vertex cover of the bipartite graph of the A-B edges (Koenig: maximum
+  // matching, then the alternating-path closure of the unmatched bA nodes)
+  S = min_cover(st, bA, bB, sB);
+  const int sS = ++st.cur;
+  for (int n : S) st.stamp[n] = sS;
+  A2.clear();
+  B2.clear();
+  for (int n : A)
+    if (st.stamp[n] != sS) A2.push_back(n);
+  for (int n : B)
+    if (st.stamp[n] != sS) B2.push_back(n);
+}
+
+// Nested dissection: the split across the axis with the smallest separator
+// (ties: the longer extent), both halves recursively, then the separator.
 void nd_rec(NDState &st, std::vector<int32_t> nodes) {
   if (nodes.empty()) return;
   if (nodes.size() <= (size_t)LEAF) {
@@ -190,45 +229,28 @@ void nd_rec(NDState &st, std::vector<int32_t> nodes) {
       lo[k] = std::min(lo[k], (*st.xyz)[3 * n + k]);
       hi[k] = std::max(hi[k], (*st.xyz)[3 * n + k]);
     }
-  int ax = 0;
-  for (int k = 1; k < 3; ++k)
-    if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
-  const size_t mid = nodes.size() / 2;
-  std::nth_element(nodes.begin(), nodes.begin() + mid, nodes.end(), [&](int a, int b) {
-    const double xa = (*st.xyz)[3 * a + ax], xb = (*st.xyz)[3 * b + ax];
-    return xa < xb || (xa == xb && a < b);
-  });
-  std::vector<int32_t> A(nodes.begin(), nodes.begin() + mid), B(nodes.begin() + mid, nodes.end());
-  const int sA = ++st.cur;
-  for (int n : A) st.stamp[n] = sA;
-  const int sB = ++st.cur;
-  for (int n : B) st.stamp[n] = sB;
-  // separator on the side with the smaller boundary
-  std::vector<int32_t> bA, bB;
-  for (int n : A) {
-    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
-      if (st.stamp[(*st.adj)[k]] == sB) { bA.push_back(n); break; }
+  int order[3] = {0, 1, 2};
+  std::sort(order, order + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+  std::vector<int32_t> A2, B2, S, bestA, bestB, bestS;
+  bool have = false;
+  for (int ax : order) {
+    if (!(hi[ax] > lo[ax])) continue;
+    split(st, nodes, ax, A2, B2, S);
+    if (!have || S.size() < bestS.size()) {
+      bestA.swap(A2);
+      bestB.swap(B2);
+      bestS.swap(S);
+      have = true;
+    }
   }
-  for (int n : B) {
-    for (int64_t k = (*st.ptr)[n]; k < (*st.ptr)[n + 1]; ++k)
-      if (st.stamp[(*st.adj)[k]] == sA) { bB.push_back(n); break; }
+  if (!have) {  // all nodes at one point: one block
+    st.blocks.push_back(nodes);
+    return;
   }
-  // smallest vertex separator between the two boundary layers: a minimum
-  // vertex cover of the bipartite graph of the A-B edges (Koenig: maximum
-  // matching, then the alternating-path closure of the unmatched bA nodes)
-  const std::vector<int32_t> S = min_cover(st, bA, bB, sB);
-  const int sS = ++st.cur;
-  for (int n : S) st.stamp[n] = sS;
-  std::vector<int32_t> A2, B2;
-  for (int n : A)
-    if (st.stamp[n] != sS) A2.push_back(n);
-  for (int n : B)
-    if (st.stamp[n] != sS) B2.push_back(n);
-  std::vector<int32_t> Sv = S;
-  spatial_order(st, Sv);
-  nd_rec(st, A2);
-  nd_rec(st, B2);
-  if (!Sv.empty()) st.blocks.push_back(Sv);
+  spatial_order(st, bestS);
+  nd_rec(st, bestA);
+  nd_rec(st, bestB);
+  if (!bestS.empty()) st.blocks.push_back(bestS);
 }
 
 // ---- device kernels --------------------------------------------------------------------
