@@ -12,16 +12,23 @@ the same denominator for every form).  Inputs are seeded synthetic fields
 6 Voronoi plates at the surface, free slip at the CMB).
 
 Forms:
-  learned (default): the headline metric.  Input is a held-out time step of
-    the seeded c1 convection trajectory (vertex temperatures of the step and
-    of the step before, shipped in m3tet_sph_amd/data).  Untimed: the FP64
-    solve of the previous step gives (U_prev, P_prev); a cold FP64 solve of
-    the step gives |r_P^cold|.  Timed, each as K solves after W warm-ups:
+  learned (default): the headline metric.  Input is a held-out time step k
+    of the seeded c1 convection trajectory (vertex temperatures and times of
+    steps k-3..k, shipped in m3tet_sph_amd/data).  Untimed: FP64 solves of
+    steps k-3..k-1 to the same rel.res as the timed solves give the
+    converged states that a time loop holds when it reaches step k; a cold
+    FP64 solve of step k gives |r_P^cold|.  Timed,
+    each as K solves after W warm-ups:
       plain FP64 MGPCG from a cold start (the reference solver as configured
         for a single solve: P = 0, U from the BCs);
-      FP64 from the previous step (the reference's time loop warm start);
-      mixed precision + learned warm start: the network's prediction from
-        (T, ln eta, U_prev, P_prev) followed by the mixed-precision solve;
+      FP64 and mixed from the previous step (the reference's time loop warm
+        start, flow3d_patera_mg_sph_p.m:204-231);
+      FP64 and mixed from the quadratic extrapolation in time of the last
+        three converged states (no network);
+      FP64 + learned, and mixed precision + learned warm start: the
+        extrapolation plus the network's correction, computed from
+        (T, ln eta, the extrapolated state, T minus its extrapolation),
+        followed by the solve;
     all stopping at the same fixed residual.  ``value`` is the mixed+learned
     rate; the others are reported beside it.
   fp64 / mixed: cold solves of the seed-0 temperature field.
@@ -249,15 +256,22 @@ def main():
             raise SystemExit(f"bench.py: trained network or held-out states missing ({wpath}, {spath}); "
                              f"run python -m m3tet_sph_amd.train_warm_start first")
         net = L.WarmStartNet(mesh)
-        net.load(wpath)
+        meta = net.load(wpath)
         z = np.load(spath, allow_pickle=False)
-        step = int(z["steps"][-1])
-        Tv = z["T"][-1].astype(np.float64)
-        Tp = z["T_prev"][-1].astype(np.float64)
-        Vp, Dp = F.material(L.nodal_temperature(mesh, Tp, net.edge_parent), vc)
-        U_prev, P_prev, _ = ctx.flow3d(Vp, Dp, phys, rtol_Pat=0.1 * args.tol, rtol_U_final=0.1 * args.tol)
+        step, order = int(z["steps"][-1]), int(z["order"])
+        if int(meta.get("order", -1)) != order:
+            raise SystemExit(f"bench.py: {wpath} was trained with order {meta.get('order')}, states have {order}")
+        T_hist, t_hist = z["T_hist"][-1], z["t_hist"][-1]
+        Tv, t_k = T_hist[-1], float(t_hist[-1])
+        net.clear_history()
+        for T_j, t_j in zip(T_hist[:-1], t_hist[:-1]):
+            # converged as a time loop converges them: to the same rel.res as the timed solves
+            Vj, Dj = F.material(L.nodal_temperature(mesh, T_j, net.edge_parent), vc)
+            U_prev, P_prev, _ = ctx.flow3d(Vj, Dj, phys, rtol_Pat=0.8 * args.tol, rtol_U_final=0.5 * args.tol)
+            net.push_state(float(t_j), T_j, U_prev, P_prev)
         Visc, Dens = F.material(L.nodal_temperature(mesh, Tv, net.edge_parent), vc)
-        workload = f"held-out step {step} of the seeded convection trajectory (seed 2)"
+        workload = (f"held-out step {step} of the seeded convection trajectory (seed 2), "
+                    f"converged states of steps {step - order - 1}..{step - 1} given")
     else:
         Visc, Dens = inp["Visc"], inp["Dens"]
 
@@ -274,8 +288,12 @@ def main():
         return ctx.flow3d(Visc, Dens, phys, P0=P0, U0=U0, precision=form, rtol_U_final=0.5 * args.tol,
                           tol_Pat_abs=tol_abs)
 
+    def extrap(form):
+        _, U0, P0 = net.extrapolate(t_k, order)
+        return warm(form, U0, P0)
+
     def learned(form):
-        U0, P0 = net.warm_start(Tv, Tp, Visc, U_prev, P_prev)
+        U0, P0 = net.warm_start_hist(t_k, order, Tv, Visc)
         return warm(form, U0, P0)
 
     def rel_res(U, P):
@@ -303,6 +321,8 @@ def main():
     if args.form == "learned":
         legs["fp64_prev_step"], _, _, _ = timed(lambda: warm("fp64", U_prev, P_prev))
         legs["mixed_prev_step"], _, _, _ = timed(lambda: warm("mixed", U_prev, P_prev))
+        legs["fp64_extrapolated"], _, _, _ = timed(lambda: extrap("fp64"))
+        legs["mixed_extrapolated"], _, _, _ = timed(lambda: extrap("mixed"))
         legs["fp64_learned"], _, _, _ = timed(lambda: learned("fp64"))
         for k, v in legs.items():
             log(f"{k}: {v}")
@@ -362,6 +382,7 @@ def main():
     if legs:
         out["legs"] = legs
         out["speedup_vs_fp64_prev_step"] = round(legs["fp64_prev_step"]["ms_per_solve"] / ms, 3)
+        out["speedup_vs_mixed_extrapolated"] = round(legs["mixed_extrapolated"]["ms_per_solve"] / ms, 3)
     print(json.dumps(out), flush=True)
 
 

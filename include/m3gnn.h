@@ -20,11 +20,14 @@
  *   y    = h_3 Wd + bd                                        n x 4  (FP32)
  * Weights and all arithmetic are FP32; activations are stored in BF16.
  * Features per vertex: T, ln(viscosity), previous (u_theta, u_phi, u_r)/s_u,
- * previous p/s_p, (r - r_in)/(r_out - r_in), T - T_prev, with s_u the RMS
- * vertex speed and s_p the RMS vertex pressure of the previous state.  The
- * output is the normalised change: U0 = U_prev + y[0:3] out_sigma[0:3] s_u
- * at vertices (edge nodes take the mean of their two vertices' changes),
- * P0 = P_prev + y[3] out_sigma[3] s_p.
+ * previous p/s_p, (r - r_in)/(r_out - r_in), (T - T_prev)/s_T, with s_u the
+ * RMS vertex speed and s_p the RMS vertex pressure of the previous state and
+ * s_T the RMS of T - T_prev.  The output is the normalised change:
+ * U0 = U_prev + y[0:3] out_sigma[0:3] s_u s_T at vertices (edge nodes take
+ * the mean of their two vertices' changes), P0 = P_prev + y[3] out_sigma[3]
+ * s_p s_T.  Scaling both the temperature change and the target by s_T makes
+ * the learned map independent of the size of the step: the change of (U, P)
+ * is close to linear in the change of T.
  *
  * Parameter vector (FP32, M3_GNN_NPARAM values): We (8x64 row-major, input
  * index first), be (64), then per layer W_l (128x64) and b_l (64), then Wd
@@ -76,7 +79,7 @@ const char *m3_gnn_last_error(const m3_gnn *g);
 int m3_gnn_set_params(m3_gnn *g, const float *params, const float *norm);
 int m3_gnn_get_params(m3_gnn *g, float *params, float *norm);
 
-/* Features (n x 8, FP32) and the scales (s_u, s_p) of one state. */
+/* Features (n x 8, FP32) and the scales (s_u, s_p, s_T) of one state. */
 int m3_gnn_features(m3_gnn *g, const double *T, const double *T_prev, const double *Visc,
                     const double *U_prev, const double *P_prev, float *X, double *scales);
 
@@ -111,6 +114,27 @@ int m3_gnn_forward_backward(m3_gnn *g, const float *X, const float *Y, float *pr
 /* Warm start for the next solve: U0 (3*nnod) and P0 (nvert). */
 int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const double *Visc,
                       const double *U_prev, const double *P_prev, double *U0, double *P0);
+
+/* Extrapolated base state.  The caller pushes each converged state of its
+ * time loop (T nvert, U 3*nnod, P nvert, at time t; times must increase);
+ * the library keeps the last M3_GNN_HIST on the device.  The base state at a
+ * new time t is the polynomial in time of degree min(order, pushed - 1)
+ * through the most recent pushed states, evaluated at t (Lagrange form):
+ * order 0 is the previous state (the reference's start), 1 the linear and
+ * 2 the quadratic extrapolation of (T, U, P).  The *_hist entry points then
+ * act as m3_gnn_add_sample / m3_gnn_warm_start with (T_prev, U_prev, P_prev)
+ * replaced by the base state, so the network predicts the converged state's
+ * departure from the extrapolation from the temperature's departure
+ * T - T_base. */
+#define M3_GNN_HIST 4
+int m3_gnn_clear_history(m3_gnn *g);
+int m3_gnn_push_state(m3_gnn *g, double t, const double *T, const double *U, const double *P);
+/* Base state at time t; any output may be NULL. */
+int m3_gnn_extrapolate(m3_gnn *g, double t, int32_t order, double *T_base, double *U_base, double *P_base);
+int m3_gnn_add_sample_hist(m3_gnn *g, int32_t idx, double t, int32_t order, const double *T,
+                           const double *Visc, const double *U_next, const double *P_next);
+int m3_gnn_warm_start_hist(m3_gnn *g, double t, int32_t order, const double *T, const double *Visc,
+                           double *U0, double *P0);
 
 #ifdef __cplusplus
 }

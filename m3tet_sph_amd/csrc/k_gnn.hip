@@ -44,27 +44,34 @@ __device__ inline float lanebc(float v, int lane) {
 inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
 // ---------------------------------------------------------------- features
-__global__ void k_scales(int64_t n, const double *U, const double *P, double *sc) {
-  double su = 0, sp = 0;
+// s_u, s_p: RMS vertex speed and pressure of the base state; s_T: RMS of the
+// temperature's departure from the base state's temperature
+__global__ void k_scales(int64_t n, const double *U, const double *P, const double *T, const double *Tb,
+                         double *sc) {
+  double su = 0, sp = 0, st = 0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     su += U[3 * i] * U[3 * i] + U[3 * i + 1] * U[3 * i + 1] + U[3 * i + 2] * U[3 * i + 2];
     sp += P[i] * P[i];
+    st += (T[i] - Tb[i]) * (T[i] - Tb[i]);
   }
-  __shared__ double a[256], b[256];
+  __shared__ double a[256], b[256], c[256];
   a[threadIdx.x] = su;
   b[threadIdx.x] = sp;
+  c[threadIdx.x] = st;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       a[threadIdx.x] += a[threadIdx.x + o];
       b[threadIdx.x] += b[threadIdx.x + o];
+      c[threadIdx.x] += c[threadIdx.x + o];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    const double u = sqrt(a[0] / (double)n), p = sqrt(b[0] / (double)n);
+    const double u = sqrt(a[0] / (double)n), p = sqrt(b[0] / (double)n), t = sqrt(c[0] / (double)n);
     sc[0] = u > 0 ? u : 1.0;
     sc[1] = p > 0 ? p : 1.0;
+    sc[2] = t > 0 ? t : 1.0;
   }
 }
 
@@ -72,7 +79,7 @@ __global__ void k_features(int64_t n, const double *T, const double *Tp, const d
                            const double *P, const float *rn, const double *sc, float *X) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double su = sc[0], sp = sc[1];
+  const double su = sc[0], sp = sc[1], st = sc[2];
   float *x = X + NF * i;
   x[0] = (float)T[i];
   x[1] = (float)log(visc[i]);
@@ -81,14 +88,14 @@ __global__ void k_features(int64_t n, const double *T, const double *Tp, const d
   x[4] = (float)(U[3 * i + 2] / su);
   x[5] = (float)(P[i] / sp);
   x[6] = rn[i];
-  x[7] = (float)(T[i] - Tp[i]);
+  x[7] = (float)((T[i] - Tp[i]) / st);
 }
 
 __global__ void k_targets(int64_t n, const double *U0, const double *P0, const double *U1, const double *P1,
                           const double *sc, float *Y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double su = sc[0], sp = sc[1];
+  const double su = sc[0] * sc[2], sp = sc[1] * sc[2];
   for (int c = 0; c < 3; ++c) Y[NO * i + c] = (float)((U1[3 * i + c] - U0[3 * i + c]) / su);
   Y[NO * i + 3] = (float)((P1[i] - P0[i]) / sp);
 }
@@ -379,7 +386,7 @@ __global__ void k_apply_vert(int64_t n, const float *Y, const float *norm, const
                              const double *Pp, double *dU, double *U0, double *P0) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double su = sc[0], sp = sc[1];
+  const double su = sc[0] * sc[2], sp = sc[1] * sc[2];
   for (int c = 0; c < 3; ++c) {
     const double d = (double)Y[NO * i + c] * (double)norm[2 * NF + c] * su;
     dU[3 * i + c] = d;
@@ -394,6 +401,24 @@ __global__ void k_apply_edge(int64_t nv, int64_t ne, const int32_t *par, const d
   if (k >= ne) return;
   const int64_t a = par[2 * k], b = par[2 * k + 1], node = nv + k;
   for (int c = 0; c < 3; ++c) U0[3 * node + c] = Up[3 * node + c] + 0.5 * (dU[3 * a + c] + dU[3 * b + c]);
+}
+
+// ---------------------------------------------------------------- history
+// out = sum_j w_j a_j (j = 0 is the most recent state), the Lagrange
+// extrapolation of the pushed states to the new time
+constexpr int NH = M3_GNN_HIST;
+struct LagIn {
+  const double *a[NH];
+  double w[NH];
+  int k;
+};
+
+__global__ void k_lagrange(int64_t n, LagIn in, double *out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = in.w[0] * in.a[0][i];
+  for (int j = 1; j < in.k; ++j) s = fma(in.w[j], in.a[j][i], s);
+  out[i] = s;
 }
 
 }  // namespace
@@ -420,6 +445,10 @@ struct m3_gnn {
   // training set
   int32_t nsample = 0;
   DBuf<float> sX, sY;
+  // converged states pushed by the caller (ring; the most recent is at head - 1)
+  DBuf<double> hT[NH], hU[NH], hP[NH];
+  double ht[NH] = {};
+  int nhist = 0, head = 0;
 };
 
 namespace {
@@ -511,7 +540,7 @@ void upload_state(m3_gnn *g, const double *T, const double *Tp, const double *Vi
   HIPCHECK(hipMemcpyAsync(g->visc.p, Visc, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(g->Up.p, Up, sizeof(double) * 3 * g->nnod, hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(g->Pp.p, Pp, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->sc.p);
+  hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
 }
 
 void features_to(m3_gnn *g, float *X) {
@@ -521,6 +550,63 @@ void features_to(m3_gnn *g, float *X) {
 
 void require_params(const m3_gnn *g) {
   M3_REQUIRE(!g->h_prm.empty(), M3_E_STATE, "network parameters not set (m3_gnn_set_params)");
+}
+
+// new temperature and viscosity of the step being started
+void upload_new(m3_gnn *g, const double *T, const double *Visc) {
+  M3_REQUIRE(T && Visc, M3_E_ARG, "null argument");
+  const int64_t n = g->n;
+  for (int64_t i = 0; i < n; ++i)
+    M3_REQUIRE(Visc[i] > 0, M3_E_NUMERIC, "Calculated a zero or negative viscosity");
+  HIPCHECK(hipMemcpyAsync(g->T.p, T, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+  HIPCHECK(hipMemcpyAsync(g->visc.p, Visc, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+}
+
+// Base state at time t in (Tp, Up, Pp): the polynomial of degree
+// min(order, nhist - 1) through the most recent pushed states, evaluated at
+// t (Lagrange form; order 0 is the last state itself), and, with the new
+// temperature already in T, the scales of the features.
+void make_base(m3_gnn *g, double t, int order, bool scales = true) {
+  M3_REQUIRE(order >= 0 && order < NH, M3_E_ARG, "extrapolation order must be 0..3");
+  M3_REQUIRE(g->nhist > 0, M3_E_STATE, "no converged state pushed (m3_gnn_push_state)");
+  M3_REQUIRE(std::isfinite(t), M3_E_ARG, "time must be finite");
+  const int k = std::min(order + 1, g->nhist);
+  LagIn lt{}, lu{}, lp{};
+  int slot[NH];
+  for (int j = 0; j < k; ++j) slot[j] = (g->head - 1 - j + 2 * NH) % NH;
+  for (int j = 0; j < k; ++j) {
+    double w = 1.0;
+    for (int m = 0; m < k; ++m)
+      if (m != j) w *= (t - g->ht[slot[m]]) / (g->ht[slot[j]] - g->ht[slot[m]]);
+    lt.w[j] = lu.w[j] = lp.w[j] = w;
+    lt.a[j] = g->hT[slot[j]].p;
+    lu.a[j] = g->hU[slot[j]].p;
+    lp.a[j] = g->hP[slot[j]].p;
+  }
+  lt.k = lu.k = lp.k = k;
+  hipStream_t s = g->stream;
+  const int64_t n = g->n, nn = g->nnod;
+  hipLaunchKernelGGL(k_lagrange, dim3(nblk(n)), dim3(256), 0, s, n, lt, g->Tp.p);
+  hipLaunchKernelGGL(k_lagrange, dim3(nblk(3 * nn)), dim3(256), 0, s, 3 * nn, lu, g->Up.p);
+  hipLaunchKernelGGL(k_lagrange, dim3(nblk(n)), dim3(256), 0, s, n, lp, g->Pp.p);
+  if (scales)  // needs the new temperature in T
+    hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
+}
+
+// U0 = Up + network change (vertices; edge nodes take the mean of their two
+// vertices' change), P0 = Pp + change; downloaded to the caller
+void apply_and_download(m3_gnn *g, double *U0, double *P0) {
+  hipStream_t s = g->stream;
+  const int64_t n = g->n, nn = g->nnod;
+  hipLaunchKernelGGL(k_apply_vert, dim3(nblk(n)), dim3(256), 0, s, n, g->Y.p, g->norm.p, g->sc.p, g->Up.p,
+                     g->Pp.p, g->dU.p, g->U0.p, g->P0.p);
+  if (nn > n)
+    hipLaunchKernelGGL(k_apply_edge, dim3(nblk(nn - n)), dim3(256), 0, s, n, nn - n, g->par.p, g->dU.p, g->Up.p,
+                       g->U0.p);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(U0, g->U0.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(P0, g->P0.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace
@@ -589,7 +675,7 @@ int m3_gnn_create(m3_gnn **out, int hip_device, const m3_graph_desc *d) {
     }
     g->part.alloc((size_t)256 * (2 * WD + 1) * WD);
     g->dpart.alloc(nblk(NO * n) + 16 * 256);
-    g->sc.alloc(2);
+    g->sc.alloc(3);
     g->dU.alloc(3 * n);
     g->dloss.alloc(1);
     for (auto *b : {&g->T, &g->Tp, &g->visc, &g->Pp, &g->Pn, &g->P0}) b->alloc(n);
@@ -651,7 +737,7 @@ int m3_gnn_features(m3_gnn *g, const double *T, const double *T_prev, const doub
     features_to(g, g->X.p);
     HIPCHECK(hipGetLastError());
     if (X) HIPCHECK(hipMemcpyAsync(X, g->X.p, sizeof(float) * NF * g->n, hipMemcpyDeviceToHost, g->stream));
-    if (scales) HIPCHECK(hipMemcpyAsync(scales, g->sc.p, sizeof(double) * 2, hipMemcpyDeviceToHost, g->stream));
+    if (scales) HIPCHECK(hipMemcpyAsync(scales, g->sc.p, sizeof(double) * 3, hipMemcpyDeviceToHost, g->stream));
     HIPCHECK(hipStreamSynchronize(g->stream));
   });
 }
@@ -811,19 +897,83 @@ int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const do
     require_params(g);
     M3_REQUIRE(U0 && P0, M3_E_ARG, "null argument");
     upload_state(g, T, T_prev, Visc, U_prev, P_prev);
-    hipStream_t s = g->stream;
-    const int64_t n = g->n, nn = g->nnod;
     features_to(g, g->X.p);
     forward(g, g->X.p);
-    hipLaunchKernelGGL(k_apply_vert, dim3(nblk(n)), dim3(256), 0, s, n, g->Y.p, g->norm.p, g->sc.p, g->Up.p,
-                       g->Pp.p, g->dU.p, g->U0.p, g->P0.p);
-    if (nn > n)
-      hipLaunchKernelGGL(k_apply_edge, dim3(nblk(nn - n)), dim3(256), 0, s, n, nn - n, g->par.p, g->dU.p, g->Up.p,
-                         g->U0.p);
+    apply_and_download(g, U0, P0);
+  });
+}
+
+int m3_gnn_clear_history(m3_gnn *g) {
+  return gguard(g, [&] {
+    g->nhist = 0;
+    g->head = 0;
+  });
+}
+
+int m3_gnn_push_state(m3_gnn *g, double t, const double *T, const double *U, const double *P) {
+  return gguard(g, [&] {
+    M3_REQUIRE(T && U && P, M3_E_ARG, "null argument");
+    M3_REQUIRE(std::isfinite(t), M3_E_ARG, "time must be finite");
+    if (g->nhist) M3_REQUIRE(t > g->ht[(g->head + NH - 1) % NH], M3_E_ARG, "times of pushed states must increase");
+    const int s = g->head;
+    const int64_t n = g->n, nn = g->nnod;
+    if (!g->hU[s].p) {
+      g->hT[s].alloc(n);
+      g->hU[s].alloc(3 * nn);
+      g->hP[s].alloc(n);
+    }
+    HIPCHECK(hipMemcpyAsync(g->hT[s].p, T, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+    HIPCHECK(hipMemcpyAsync(g->hU[s].p, U, sizeof(double) * 3 * nn, hipMemcpyHostToDevice, g->stream));
+    HIPCHECK(hipMemcpyAsync(g->hP[s].p, P, sizeof(double) * n, hipMemcpyHostToDevice, g->stream));
+    HIPCHECK(hipStreamSynchronize(g->stream));
+    g->ht[s] = t;
+    g->head = (s + 1) % NH;
+    g->nhist = std::min(g->nhist + 1, NH);
+  });
+}
+
+int m3_gnn_extrapolate(m3_gnn *g, double t, int32_t order, double *T_base, double *U_base, double *P_base) {
+  return gguard(g, [&] {
+    make_base(g, t, order, false);
+    hipStream_t s = g->stream;
+    const int64_t n = g->n, nn = g->nnod;
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(U0, g->U0.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(P0, g->P0.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (T_base) HIPCHECK(hipMemcpyAsync(T_base, g->Tp.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    if (U_base) HIPCHECK(hipMemcpyAsync(U_base, g->Up.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+    if (P_base) HIPCHECK(hipMemcpyAsync(P_base, g->Pp.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+int m3_gnn_add_sample_hist(m3_gnn *g, int32_t idx, double t, int32_t order, const double *T, const double *Visc,
+                           const double *U_next, const double *P_next) {
+  return gguard(g, [&] {
+    M3_REQUIRE(idx >= 0 && idx < g->nsample, M3_E_ARG, "sample index out of range (m3_gnn_reserve)");
+    M3_REQUIRE(U_next && P_next, M3_E_ARG, "null argument");
+    upload_new(g, T, Visc);
+    make_base(g, t, order);
+    hipStream_t s = g->stream;
+    const int64_t n = g->n;
+    HIPCHECK(hipMemcpyAsync(g->Un.p, U_next, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(g->Pn.p, P_next, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    features_to(g, g->sX.p + (size_t)idx * NF * n);
+    hipLaunchKernelGGL(k_targets, dim3(nblk(n)), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->Un.p, g->Pn.p, g->sc.p,
+                       g->sY.p + (size_t)idx * NO * n);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(s));
+  });
+}
+
+int m3_gnn_warm_start_hist(m3_gnn *g, double t, int32_t order, const double *T, const double *Visc, double *U0,
+                           double *P0) {
+  return gguard(g, [&] {
+    require_params(g);
+    M3_REQUIRE(U0 && P0, M3_E_ARG, "null argument");
+    upload_new(g, T, Visc);
+    make_base(g, t, order);
+    features_to(g, g->X.p);
+    forward(g, g->X.p);
+    apply_and_download(g, U0, P0);
   });
 }
 

@@ -56,9 +56,15 @@ LIB.m3_gnn_eval.argtypes = [C.c_void_p, I32, C.c_int32, F32]
 LIB.m3_gnn_sample_stats.argtypes = [C.c_void_p, I32, C.c_int32, D]
 LIB.m3_gnn_forward_backward.argtypes = [C.c_void_p, F32, F32, F32, F32, F32]
 LIB.m3_gnn_warm_start.argtypes = [C.c_void_p, D, D, D, D, D, D, D]
+LIB.m3_gnn_clear_history.argtypes = [C.c_void_p]
+LIB.m3_gnn_push_state.argtypes = [C.c_void_p, C.c_double, D, D, D]
+LIB.m3_gnn_extrapolate.argtypes = [C.c_void_p, C.c_double, C.c_int32, D, D, D]
+LIB.m3_gnn_add_sample_hist.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, D, D, D, D]
+LIB.m3_gnn_warm_start_hist.argtypes = [C.c_void_p, C.c_double, C.c_int32, D, D, D, D]
 for _f in ("m3_gnn_create", "m3_gnn_set_params", "m3_gnn_get_params", "m3_gnn_features", "m3_gnn_reserve",
            "m3_gnn_add_sample", "m3_gnn_fit_norm", "m3_gnn_train", "m3_gnn_eval", "m3_gnn_forward_backward",
-           "m3_gnn_warm_start", "m3_gnn_sample_stats"):
+           "m3_gnn_warm_start", "m3_gnn_sample_stats", "m3_gnn_clear_history", "m3_gnn_push_state",
+           "m3_gnn_extrapolate", "m3_gnn_add_sample_hist", "m3_gnn_warm_start_hist"):
     getattr(LIB, _f).restype = C.c_int
 
 
@@ -162,7 +168,7 @@ class WarmStartNet:
 
     def features(self, T, T_prev, Visc, U_prev, P_prev):
         X = np.zeros((self.n, NF), dtype=np.float32)
-        sc = np.zeros(2)
+        sc = np.zeros(3)
         a = [self._d(v) for v in (T, T_prev, Visc, U_prev, P_prev)]
         self._check(LIB.m3_gnn_features(self._h, *[_p(v) for v in a], _p(X, F32), _p(sc)))
         return X, sc
@@ -213,6 +219,35 @@ class WarmStartNet:
         self._check(LIB.m3_gnn_warm_start(self._h, *[_p(v) for v in a], _p(U0), _p(P0)))
         return U0, P0
 
+    # -- time history: extrapolated base states (include/m3gnn.h)
+    def clear_history(self):
+        self._check(LIB.m3_gnn_clear_history(self._h))
+
+    def push_state(self, t, T, U, P):
+        """Record the converged state (T at vertices, U, P) of time t."""
+        a = [self._d(v) for v in (T, U, P)]
+        self._check(LIB.m3_gnn_push_state(self._h, float(t), *[_p(v) for v in a]))
+
+    def extrapolate(self, t, order=2):
+        """Base state (T, U, P) at time t: Lagrange extrapolation of degree
+        min(order, pushed - 1) through the most recent pushed states."""
+        Tb, Ub, Pb = np.zeros(self.n), np.zeros(3 * self.mesh.nnod), np.zeros(self.n)
+        self._check(LIB.m3_gnn_extrapolate(self._h, float(t), int(order), _p(Tb), _p(Ub), _p(Pb)))
+        return Tb, Ub, Pb
+
+    def add_sample_hist(self, idx, t, order, T, Visc, U_next, P_next):
+        a = [self._d(v) for v in (T, Visc, U_next, P_next)]
+        self._check(LIB.m3_gnn_add_sample_hist(self._h, int(idx), float(t), int(order), *[_p(v) for v in a]))
+
+    def warm_start_hist(self, t, order, T, Visc):
+        """Network start from the extrapolated base state at time t."""
+        a = [self._d(v) for v in (T, Visc)]
+        U0 = np.zeros(3 * self.mesh.nnod)
+        P0 = np.zeros(self.n)
+        self._check(LIB.m3_gnn_warm_start_hist(self._h, float(t), int(order), *[_p(v) for v in a], _p(U0),
+                                               _p(P0)))
+        return U0, P0
+
     def save(self, path, meta=None):
         p, q = self.get_params()
         np.savez_compressed(path, params=p, norm=q, meta=json.dumps(meta or {}))
@@ -249,22 +284,27 @@ def time_step(U, nV, h_min, cfl=0.5):
 
 
 def generate_trajectory(mesh, ctx, net, nstates, visc_contrast=1e4, Ra=1e5, seed=2, cfl=0.5, tol=1e-7,
-                        keep_from=None, log=print):
+                        keep_from=None, order=2, plume=None, log=print):
     """Plain FP64 solves along a semi-Lagrangian convection trajectory.
-    Sample k-1 of ``net``'s training set (k = 1..nstates-1) maps state k-1 and
-    the temperature of state k to the converged state k.  Returns per-state
-    temperatures, time steps, solver statistics and the full (U, P) of the
-    states with index >= keep_from."""
+    Sample k-1 of ``net``'s training set (k = 1..nstates-1) maps the base
+    state of step k (the degree-``order`` extrapolation in time of the
+    converged states before it, pushed into ``net``'s history) and the
+    temperature of state k to the converged state k.  Each solve starts from
+    that base state and stops at a fixed residual (``tol`` relative to the
+    cold pressure residual of state 0 and to |f_U|).  Returns per-state
+    temperatures, times, time steps, solver statistics and the full (U, P) of
+    the states with index >= keep_from."""
     ptr, col, parent, _ = net.adj_ptr, net.adj_col, net.edge_parent, net.radius
     h_min = min_edge_length(mesh, ptr, col)
     phys = fields.physics(Ra)
     nV = mesh.nVnod
-    Tv = fields.temperature(mesh, seed=seed)[:nV].copy()
+    Tv = fields.temperature(mesh, seed=seed, plume=plume)[:nV].copy()
     keep_from = nstates if keep_from is None else keep_from
-    Ts, dts, stats, kept = [], [], [], {}
-    U_prev = P_prev = T_prev = None
+    Ts, ts, dts, stats, kept = [], [], [], [], {}
     rP_cold = None
     net.reserve(max(nstates - 1, 1))
+    net.clear_history()
+    t_now = 0.0
     t0 = time.time()
     for k in range(nstates):
         Visc, Dens = fields.material(nodal_temperature(mesh, Tv, parent), visc_contrast)
@@ -272,21 +312,24 @@ def generate_trajectory(mesh, ctx, net, nstates, visc_contrast=1e4, Ra=1e5, seed
             U, P, info = ctx.flow3d(Visc, Dens, phys, rtol_Pat=tol, rtol_U_final=tol)
             rP_cold = info["norm_rP0"]
         else:
-            U, P, info = ctx.flow3d(Visc, Dens, phys, P0=P_prev, U0=U_prev, rtol_U_final=tol,
-                                    tol_Pat_abs=tol * rP_cold)
-            net.add_sample(k - 1, Tv, T_prev, Visc, U_prev, P_prev, U, P)
+            _, Ub, Pb = net.extrapolate(t_now, order)
+            U, P, info = ctx.flow3d(Visc, Dens, phys, P0=Pb, U0=Ub, rtol_U_final=tol, tol_Pat_abs=tol * rP_cold)
+            net.add_sample_hist(k - 1, t_now, order, Tv, Visc, U, P)
+        net.push_state(t_now, Tv, U, P)
         Ts.append(Tv.copy())
+        ts.append(t_now)
         stats.append(dict(nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"]))
-        if k >= keep_from - 1:
+        if k >= keep_from:
             kept[k] = (U.copy(), P.copy())
         dt = time_step(U, nV, h_min, cfl)
         dts.append(dt)
         if k % 25 == 0 or k == nstates - 1:
             log(f"[trajectory] state {k}/{nstates}: nitP {info['nitP']} nitZ {info['nitZ']} dt {dt:.3e} "
                 f"Tmean {Tv.mean():.4f} ({time.time() - t0:.1f} s)", flush=True)
-        T_prev, U_prev, P_prev = Tv, U, P
         Tv = ctx.sl_advect(Tv, U, dt, 1.0, 0.0)
-    return dict(T=np.array(Ts), dt=np.array(dts), stats=stats, kept=kept, rP_cold0=rP_cold, h_min=h_min)
+        t_now += dt
+    return dict(T=np.array(Ts), t=np.array(ts), dt=np.array(dts), stats=stats, kept=kept, rP_cold0=rP_cold,
+                h_min=h_min, order=order)
 
 
 def train(net, ntrain, nval_from, nval, epochs=100, lr=1e-3, lr_min=1e-5, seed=0, ncal=50, log=print):

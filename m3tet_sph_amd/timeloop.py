@@ -1,12 +1,14 @@
 """Consecutive time steps of the convection loop with the Stokes hot path:
 solve (mixed-precision V-cycle, started from the previous step), CFL time
 step, semi-Lagrangian temperature update (with --kappa: diffusion over dt/2,
-advection over dt, diffusion over dt/2); optionally, at every step, a
-second solve of the same state started from the warm-start network, so the
-two starts are compared on identical inputs.
+advection over dt, diffusion over dt/2); optionally, at every step, more
+solves of the same state started from the extrapolation in time of the last
+converged states (--extrap ORDER) and from the warm-start network (--net,
+which adds its correction to that extrapolation), so the starts are compared
+on identical inputs.
 
     python -m m3tet_sph_amd.timeloop --config c4 --steps 50 --out <dir> \
-        [--net m3tet_sph_amd/data/warm_start_c1.npz]
+        [--extrap 2] [--net m3tet_sph_amd/data/warm_start_c1.npz]
 
 Per step it records iterations, seconds and the fixed-denominator residual
 max(|r_P|/|r_P^cold|, |r_U|/|f_U|) (r_P^cold from the cold solve of step 0),
@@ -36,6 +38,9 @@ def main(argv=None):
     ap.add_argument("--Ra", type=float, default=1e5)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--net", default=None, help="warm-start network (.npz) to compare against")
+    ap.add_argument("--extrap", type=int, default=-1,
+                    help="also solve from the extrapolation of this degree in time (0: previous state); "
+                         "also the base of --net (default: the network's training order)")
     ap.add_argument("--kappa", type=float, default=0.0,
                     help="thermal diffusivity: > 0 adds diffusion over dt/2 before and after the "
                          "advection (thermal_advdiff_3d_p.m:13-52)")
@@ -55,10 +60,15 @@ def main(argv=None):
     ctx.set_bc(ifix, vfix)
     ptr, col, parent, _ = L.vertex_graph(m)
     h_min = L.min_edge_length(m, ptr, col)
-    net = None
-    if a.net:
-        net = L.WarmStartNet(m)
-        net.load(a.net)
+    net = hist = None
+    order = a.extrap
+    if a.net or a.extrap >= 0:
+        hist = L.WarmStartNet(m)  # holds the converged-state history on the device
+        if a.net:
+            meta = hist.load(a.net)
+            net = hist
+            if order < 0:
+                order = int(meta.get("order", 0))
     log(f"[setup] {a.config}: {3 * m.nnod} velocity dofs, contrast {vc:.0e} ({time.time() - t0:.1f} s)")
     phys = fields.physics(a.Ra)
     nV = m.nVnod
@@ -73,7 +83,7 @@ def main(argv=None):
     frames = fields.rotation_frames(m.GCOORD) if a.snapshots else None
     SET, COMM = {"outdir": a.out}, {"prefix": a.config}
     iplot = 0
-    U = P = Tp = None
+    U = P = None
     if a.kappa > 0:
         ctx.diffusion_mesh()
         tfix = np.nonzero(np.isin(m.PointID[:nV], (301, 306)))[0]
@@ -91,20 +101,27 @@ def main(argv=None):
             rP_cold = info["norm_rP0"]
         else:
             tol_abs = 0.8 * a.tol * rP_cold
+            starts = []
+            if a.extrap >= 0:
+                starts.append(("extrap", lambda: hist.extrapolate(time_, order)[1:]))
             if net is not None:
+                starts.append(("net", lambda: net.warm_start_hist(time_, order, Tv, V)))
+            for name, start in starts:
                 t = time.perf_counter()
-                U0, P0 = net.warm_start(Tv, Tp, V, U, P)
+                U0, P0 = start()
                 Un, Pn, i2 = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, precision=a.form, rtol_U_final=0.5 * a.tol,
                                         tol_Pat_abs=tol_abs)
                 rU, fU, rP = api.residual(ctx, Un, Pn)
-                row["net"] = dict(s=time.perf_counter() - t, nitP=i2["nitP"], nitZ=i2["nitZ"], nitU=i2["nitU"],
-                                  rel_res=max(rU / fU, rP / rP_cold))
+                row[name] = dict(s=time.perf_counter() - t, nitP=i2["nitP"], nitZ=i2["nitZ"], nitU=i2["nitU"],
+                                 rel_res=max(rU / fU, rP / rP_cold))
             t = time.perf_counter()
             U, P, info = ctx.flow3d(V, Dn, phys, P0=P, U0=U, precision=a.form, rtol_U_final=0.5 * a.tol,
                                     tol_Pat_abs=tol_abs)
             row["prev"] = dict(s=time.perf_counter() - t, nitP=info["nitP"], nitZ=info["nitZ"], nitU=info["nitU"])
         rU, fU, rP = api.residual(ctx, U, P)
         row["rel_res"] = max(rU / fU, rP / rP_cold)
+        if hist is not None:
+            hist.push_state(time_, Tv, U, P)
         dt = L.time_step(U, nV, h_min, 0.5)
         row["dt"] = dt
         rows.append(row)
@@ -118,7 +135,6 @@ def main(argv=None):
             matio.write_output_file(SET, VAR, COMM, time_, k, dt, iplot)
             matio.write_restart_file(SET, VAR, COMM, k, iplot, time_, dt, 1 + (iplot - 1) % 2)
         time_ += dt
-        Tp = Tv
         if a.kappa > 0:
             t = time.perf_counter()
             Tv, d1 = ctx.diffusion3d(Tv, a.kappa, dt / 2, tfix, tval)
@@ -148,11 +164,12 @@ def main(argv=None):
     if a.kappa > 0:
         summ["thermal"] = dict(kappa=a.kappa, s=float(np.mean([r["thermal"]["s"] for r in rows])),
                                nit=float(np.mean([sum(r["thermal"]["nit"]) for r in rows])))
-    if net is not None:
-        summ["net"] = dict(s=float(np.mean([r["net"]["s"] for r in steps])),
-                           nitP=float(np.mean([r["net"]["nitP"] for r in steps])),
-                           nitZ=float(np.mean([r["net"]["nitZ"] for r in steps])),
-                           max_rel_res=float(max(r["net"]["rel_res"] for r in steps)))
+    for name in ("extrap", "net"):
+        if steps and name in steps[0]:
+            summ[name] = dict(s=float(np.mean([r[name]["s"] for r in steps])),
+                              nitP=float(np.mean([r[name]["nitP"] for r in steps])),
+                              nitZ=float(np.mean([r[name]["nitZ"] for r in steps])),
+                              max_rel_res=float(max(r[name]["rel_res"] for r in steps)), order=order)
     with open(os.path.join(a.out, f"timeloop_{a.config}.json"), "w") as f:
         json.dump(dict(summary=summ, per_step=rows, seconds=time.time() - t0), f, indent=1)
     log("[summary] " + json.dumps(summ))

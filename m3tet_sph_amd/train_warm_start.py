@@ -5,14 +5,15 @@ GPU and evaluate it on held-out time steps.
         --ntrain 400 --epochs 100 --out <dir>
 
 Trajectory: state 0 from the seeded temperature (seed 2), then for each
-state a plain FP64 solve (started, as in the reference's time loop, from the
-previous state) to a fixed residual, dt from CFL 0.5 on the vertex speeds,
-and the semi-Lagrangian temperature update.  Sample k-1 maps state k-1 and
-the temperature of state k to state k; samples [0, ntrain) train the
-network, the rest are held out.  For each held-out step the solve is run
-cold, from the previous state and from the network's prediction, in FP64
-and in the mixed-precision V-cycle, all to the same fixed residual
-(|r_U| <= 1e-6 |f_U| and |r_P| <= 1e-6 |r_P^cold|).
+state a plain FP64 solve to a fixed residual, dt from CFL 0.5 on the vertex
+speeds, and the semi-Lagrangian temperature update.  Sample k-1 maps the
+base state of step k (the quadratic extrapolation in time of the converged
+states k-3..k-1, --order) and the temperature of state k to state k;
+samples [0, ntrain) train the network, the rest are held out.  For each
+held-out step the solve is run cold, from the previous state (the
+reference's start), from the extrapolated base state and from the network's
+prediction, in FP64 and in the mixed-precision V-cycle, all to the same
+fixed residual (|r_U| <= 1e-6 |f_U| and |r_P| <= 1e-6 |r_P^cold|).
 
 Writes <out>/warm_start_<config>.npz (parameters, normalisation, metadata),
 <out>/<config>_heldout_states.npz (vertex temperatures of a few held-out
@@ -33,12 +34,20 @@ def log(*a, **k):
     print(*a, **k)
 
 
-def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol):
+def push_history(net, traj, k, order):
+    """Fill ``net``'s history with the converged states before step k."""
+    net.clear_history()
+    for j in range(max(0, k - 1 - order), k):
+        U, P = traj["kept"][j]
+        net.push_state(traj["t"][j], traj["T"][j], U, P)
+
+
+def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol, order):
     phys = fields.physics(Ra)
     rows = []
     t0 = time.time()
     for k in steps:
-        T, Tp = traj["T"][k], traj["T"][k - 1]
+        T, t_k = traj["T"][k], traj["t"][k]
         Up, Pp = traj["kept"][k - 1]
         V, Dn = fields.material(L.nodal_temperature(m, T, net.edge_parent), visc_contrast)
         row = dict(step=int(k))
@@ -60,32 +69,40 @@ def evaluate(m, ctx, net, traj, steps, visc_contrast, Ra, tol):
         rP_cold = ic["norm_rP0"]
         row["cold_fp64"]["rel_res"] = max(row["cold_fp64"]["rP"] / rP_cold, row["cold_fp64"]["rU_rel"])
         tol_abs = 0.8 * tol * rP_cold  # margin: the final velocity solve moves r_P slightly
+        push_history(net, traj, k, order)
         t = time.perf_counter()
-        U0, P0 = net.warm_start(T, Tp, V, Up, Pp)
+        _, Ue, Pe = net.extrapolate(t_k, order)
+        t_ext = time.perf_counter() - t
+        t = time.perf_counter()
+        U0, P0 = net.warm_start_hist(t_k, order, T, V)
         t_net = time.perf_counter() - t
-        row["t_network_s"] = t_net
-        for name, prec, u0, p0, ex in (("prev_fp64", "fp64", Up, Pp, 0.0), ("net_fp64", "fp64", U0, P0, t_net),
-                                       ("prev_mixed", "mixed", Up, Pp, 0.0), ("net_mixed", "mixed", U0, P0, t_net)):
+        row["t_extrapolate_s"], row["t_network_s"] = t_ext, t_net
+        for name, prec, u0, p0, ex in (("prev_fp64", "fp64", Up, Pp, 0.0), ("extrap_fp64", "fp64", Ue, Pe, t_ext),
+                                       ("net_fp64", "fp64", U0, P0, t_net), ("prev_mixed", "mixed", Up, Pp, 0.0),
+                                       ("extrap_mixed", "mixed", Ue, Pe, t_ext),
+                                       ("net_mixed", "mixed", U0, P0, t_net)):
             run(name, prec, u0, p0, tol_abs, ex)
             row[name]["rel_res"] = max(row[name]["rP"] / rP_cold, row[name]["rU_rel"])
         # error of the initial guesses against the converged FP64 solution
         Uc, Pc, _ = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, rtol_U_final=1e-3 * tol, tol_Pat_abs=1e-3 * tol_abs)
-        row["init_err"] = dict(prev_U=float(np.linalg.norm(Up - Uc) / np.linalg.norm(Uc)),
-                               net_U=float(np.linalg.norm(U0 - Uc) / np.linalg.norm(Uc)),
-                               prev_P=float(np.linalg.norm(Pp - Pc) / np.linalg.norm(Pc)),
-                               net_P=float(np.linalg.norm(P0 - Pc) / np.linalg.norm(Pc)))
+        nu, npr = np.linalg.norm(Uc), np.linalg.norm(Pc)
+        row["init_err"] = dict(prev_U=float(np.linalg.norm(Up - Uc) / nu), extrap_U=float(np.linalg.norm(Ue - Uc) / nu),
+                               net_U=float(np.linalg.norm(U0 - Uc) / nu), prev_P=float(np.linalg.norm(Pp - Pc) / npr),
+                               extrap_P=float(np.linalg.norm(Pe - Pc) / npr),
+                               net_P=float(np.linalg.norm(P0 - Pc) / npr))
         rows.append(row)
         if len(rows) % 10 == 1 or k == steps[-1]:
             log(f"[eval] step {k}: nitP cold {row['cold_fp64']['nitP']} prev {row['prev_fp64']['nitP']} "
-                f"net {row['net_fp64']['nitP']} | init err U prev {row['init_err']['prev_U']:.2e} "
+                f"extrap {row['extrap_fp64']['nitP']} net {row['net_fp64']['nitP']} | init err U prev "
+                f"{row['init_err']['prev_U']:.2e} extrap {row['init_err']['extrap_U']:.2e} "
                 f"net {row['init_err']['net_U']:.2e} ({time.time() - t0:.1f} s)")
-    keys = ("cold_fp64", "prev_fp64", "net_fp64", "prev_mixed", "net_mixed")
+    keys = ("cold_fp64", "prev_fp64", "extrap_fp64", "net_fp64", "prev_mixed", "extrap_mixed", "net_mixed")
     summary = {}
     for kk in keys:
         summary[kk] = {f: float(np.mean([r[kk][f] for r in rows])) for f in ("nitP", "nitZ", "nitU", "s", "rel_res")}
         summary[kk]["max_rel_res"] = float(np.max([r[kk]["rel_res"] for r in rows]))
     summary["init_err"] = {f: float(np.mean([r["init_err"][f] for r in rows]))
-                           for f in ("prev_U", "net_U", "prev_P", "net_P")}
+                           for f in ("prev_U", "extrap_U", "net_U", "prev_P", "extrap_P", "net_P")}
     return rows, summary
 
 
@@ -101,6 +118,10 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--save-steps", type=int, default=3)
+    ap.add_argument("--order", type=int, default=2, help="degree of the extrapolated base state (0: previous)")
+    ap.add_argument("--eval-steps", type=int, default=0, help="evaluate this many held-out steps (0: all)")
+    ap.add_argument("--plume", action="store_true",
+                    help="add the Gaussian CMB plume (dT 0.3, radius 0.05, theta 0.9, phi 1.3)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
@@ -111,23 +132,28 @@ def main(argv=None):
     ifix, vfix = fields.plate_bcs(m, seed=1)
     ctx.set_bc(ifix, vfix)
     net = L.WarmStartNet(m)
+    plume = (0.3, 0.05, 0.9, 1.3) if a.plume else None
     traj = L.generate_trajectory(m, ctx, net, a.states, a.visc_contrast, a.Ra, a.seed, 0.5, 0.1 * a.tol,
-                                 keep_from=a.ntrain, log=log)
+                                 keep_from=max(a.ntrain - a.order - 1, 0), order=a.order, plume=plume, log=log)
     nsample = a.states - 1
     hist = L.train(net, a.ntrain, a.ntrain, nsample - a.ntrain, a.epochs, a.lr, log=log)
     meta = dict(config=a.config, states=a.states, ntrain=a.ntrain, epochs=a.epochs, lr=a.lr, seed=a.seed,
-                visc_contrast=a.visc_contrast, Ra=a.Ra, history=hist)
+                visc_contrast=a.visc_contrast, Ra=a.Ra, order=a.order, plume=bool(a.plume), history=hist)
     net.save(os.path.join(a.out, f"warm_start_{a.config}.npz"), meta)
     held = list(range(a.ntrain + 1, a.states))
-    rows, summary = evaluate(m, ctx, net, traj, held, a.visc_contrast, a.Ra, a.tol)
+    if a.eval_steps:
+        held = [int(k) for k in np.linspace(held[0], held[-1], min(a.eval_steps, len(held))).round()]
+    rows, summary = evaluate(m, ctx, net, traj, held, a.visc_contrast, a.Ra, a.tol, a.order)
     save = np.linspace(held[0], held[-1], min(a.save_steps, len(held))).round().astype(int)
-    # FP32 copies keep the file small; they are the inputs of the held-out test and the benchmark
-    np.savez_compressed(os.path.join(a.out, f"{a.config}_heldout_states.npz"), steps=save,
-                        T=traj["T"][save].astype(np.float32), T_prev=traj["T"][save - 1].astype(np.float32),
-                        dt=traj["dt"][save - 1])
+    # inputs of the held-out test and the benchmark: vertex temperatures and
+    # times of each saved step and of the order + 1 steps before it (FP64:
+    # the network reads the small difference between T and its extrapolation)
+    lag = np.arange(-a.order - 1, 1)
+    np.savez_compressed(os.path.join(a.out, f"{a.config}_heldout_states.npz"), steps=save, order=a.order,
+                        T_hist=traj["T"][save[:, None] + lag[None, :]], t_hist=traj["t"][save[:, None] + lag[None, :]])
     out = dict(meta=meta, summary=summary, per_step=rows,
                trajectory=dict(dt=traj["dt"].tolist(), stats=traj["stats"], rP_cold0=traj["rP_cold0"],
-                               h_min=traj["h_min"]),
+                               h_min=traj["h_min"], order=traj["order"]),
                seconds=time.time() - t0)
     with open(os.path.join(a.out, f"{a.config}_warm_start_eval.json"), "w") as f:
         json.dump(out, f, indent=1)

@@ -148,30 +148,56 @@ def loss_and_grad(X, Yt, params, norm, ptr, col, exact=True):
 
 
 def features(T, T_prev, Visc, U_prev, P_prev, rn):
-    """Per-vertex features and the state scales (s_u, s_p) of include/m3gnn.h."""
+    """Per-vertex features and the state scales (s_u, s_p, s_T) of
+    include/m3gnn.h."""
     n = len(T)
     Uv = np.asarray(U_prev[:3 * n]).reshape(n, 3)
+    dT = np.asarray(T) - np.asarray(T_prev)
     su = np.sqrt(np.sum(Uv ** 2) / n)
     sp = np.sqrt(np.sum(np.asarray(P_prev) ** 2) / n)
+    st = np.sqrt(np.sum(dT ** 2) / n)
     su = su if su > 0 else 1.0
     sp = sp if sp > 0 else 1.0
+    st = st if st > 0 else 1.0
     X = np.zeros((n, NF), dtype=f32)
     X[:, 0] = T
     X[:, 1] = np.log(Visc[:n])
     X[:, 2:5] = Uv / su
     X[:, 5] = np.asarray(P_prev) / sp
     X[:, 6] = rn
-    X[:, 7] = np.asarray(T) - np.asarray(T_prev)
-    return X, np.array([su, sp])
+    X[:, 7] = dT / st
+    return X, np.array([su, sp, st])
 
 
 def warm_start(Y, norm, scales, U_prev, P_prev, parent):
     """U0, P0 from the normalised prediction Y (n x 4)."""
     n = Y.shape[0]
     osig = np.asarray(norm, dtype=f64)[2 * NF:]
-    dU = Y[:, :3].astype(f64) * osig[:3] * scales[0]
+    dU = Y[:, :3].astype(f64) * osig[:3] * (scales[0] * scales[2])
     U0 = np.array(U_prev, dtype=f64)
     U0[:3 * n] += dU.ravel()
     U0[3 * n:] += (0.5 * (dU[parent[:, 0]] + dU[parent[:, 1]])).ravel()
-    P0 = np.asarray(P_prev, dtype=f64) + Y[:, 3].astype(f64) * osig[3] * scales[1]
+    P0 = np.asarray(P_prev, dtype=f64) + Y[:, 3].astype(f64) * osig[3] * (scales[1] * scales[2])
     return U0, P0
+
+
+def extrapolate(times, states, t, order):
+    """Base state of the history entry points (include/m3gnn.h): the
+    polynomial in time of degree min(order, len(times) - 1) through the most
+    recent states (``times`` increasing, ``states`` in the same order),
+    evaluated at ``t`` in Lagrange form; summed from the most recent state
+    back, as the device does."""
+    k = min(order + 1, len(times))
+    ts = [float(x) for x in times[::-1][:k]]
+    xs = [np.asarray(x, dtype=f64) for x in states[::-1][:k]]
+    w = []
+    for j in range(k):
+        wj = 1.0
+        for m in range(k):
+            if m != j:
+                wj *= (t - ts[m]) / (ts[j] - ts[m])
+        w.append(wj)
+    out = w[0] * xs[0]
+    for j in range(1, k):
+        out = out + w[j] * xs[j]
+    return out

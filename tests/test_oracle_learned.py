@@ -126,6 +126,30 @@ def _interior(X):
     return (r > r.min() + 1e-6) & (r < r.max() - 1e-6)
 
 
+def test_extrapolation_is_exact_for_polynomials_of_its_degree():
+    """The base state of degree d through d+1 past states reproduces any
+    polynomial in time of degree <= d at a new time (uneven steps), order 0
+    returns the last state, and a degree-(d+1) polynomial is missed by the
+    leading term of the interpolation error, prod (t - t_j) times the
+    coefficient."""
+    rng = np.random.default_rng(3)
+    ts = np.cumsum(rng.uniform(0.5, 1.5, 5))
+    t = ts[-1] + 0.8
+    c = rng.standard_normal((5, 7))
+    for order in range(4):
+        poly = lambda tt, d=order: sum(c[i] * tt ** i for i in range(d + 1))
+        states = [poly(tj) for tj in ts]
+        assert np.allclose(G.extrapolate(ts, states, t, order), poly(t), rtol=1e-11, atol=1e-11)
+        # one degree more: error = c_{d+1} prod over the used points (t - t_j)
+        poly2 = lambda tt, d=order: sum(c[i] * tt ** i for i in range(d + 2))
+        err = poly2(t) - G.extrapolate(ts, [poly2(tj) for tj in ts], t, order)
+        assert np.allclose(err, c[order + 1] * np.prod(t - ts[-order - 1:]), rtol=1e-9, atol=1e-9)
+    assert np.array_equal(G.extrapolate(ts, [c[i] for i in range(5)], t, 0), c[4])
+    # fewer states than order + 1: the degree drops to what the history holds
+    assert np.allclose(G.extrapolate(ts[:2], [c[0], c[1]], t, 3),
+                       c[1] + (c[1] - c[0]) * (t - ts[1]) / (ts[1] - ts[0]), rtol=1e-13)
+
+
 def test_sl_zero_velocity_is_identity_with_boundary_values(tiny):
     X, tets = _shell_p1(tiny)
     T = np.random.default_rng(0).random(len(X))
