@@ -281,11 +281,18 @@ __device__ inline void mf_zero(MfAcc &m) {
 __device__ inline void tile_mm_mf(const double *A, const double *B, bool transB, double (*sA)[TS + 1],
                                   double (*sB)[TS + 1], MfAcc &m) {
   __syncthreads();
-  for (int t = threadIdx.x; t < TT; t += 256) {
-    const int r = t & 63, c = t >> 6;
-    sA[r][c] = A[t];
-    if (transB) sB[r][c] = B[t];
-    else sB[c][r] = B[t];
+  double va[TT / 256], vb[TT / 256];  // all 32 loads in flight before the LDS stores
+#pragma unroll
+  for (int q = 0; q < TT / 256; ++q) {
+    va[q] = A[threadIdx.x + 256 * q];
+    vb[q] = B[threadIdx.x + 256 * q];
+  }
+#pragma unroll
+  for (int q = 0; q < TT / 256; ++q) {
+    const int t = threadIdx.x + 256 * q, r = t & 63, c = t >> 6;
+    sA[r][c] = va[q];
+    if (transB) sB[r][c] = vb[q];
+    else sB[c][r] = vb[q];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -345,61 +352,92 @@ __device__ inline double bcast_lane(double v, int lane) {
 }
 
 // Cholesky of the diagonal tile and inverse of the factor, one wavefront per
-// tile.  Lane t keeps row t of the tile in registers; step k broadcasts
-// L[k][k] and the column entries L[c][k] from their owning lanes
-// (v_readlane), so the elimination needs no LDS traffic and no barriers.
-// Only the lower triangle is meaningful; the upper part of each lane's row is
-// never read.  The inverse is then formed column by column (lane j, column j)
-// by forward substitution against L staged once in LDS.
+// tile, in column blocks of PB.  Lane t owns row t.  For block kb the lane
+// keeps its row's PB columns in registers; step j broadcasts L[kb+j][kb+j]
+// and the entries L[c][kb+j] from their owning lanes (v_readlane), then the
+// columns right of the block are updated in LDS, one column per step of a
+// loop.  Every entry receives its updates in ascending column order, the
+// order of the unblocked right-looking elimination, with the same fused
+// multiply-adds; the loops over blocks and columns stay loops, which keeps
+// the kernel small (the fully unrolled elimination was ~220 KB of code, far
+// beyond the instruction cache).  Only the lower triangle is
+// meaningful; the upper part is never read.  The inverse is then formed by
+// blocked forward substitution, lane t computing column t (accumulators in
+// LDS, L read by broadcast), again in ascending column order.
+constexpr int PB = 8;
+
 __global__ __launch_bounds__(64) void k_potrf(const int32_t *cols, int col0, const int32_t *diagtile,
                                               const int32_t *pos2dof, double *tiles, double *dinv,
                                               double *dinvT, int *err) {
   const int J = cols[col0 + blockIdx.x];
   double *A = tiles + (int64_t)diagtile[J] * TT;
-  __shared__ double S[TS][TS + 1];
+  __shared__ double S[TS][TS + 1];  // L, row-major, padded
+  __shared__ double X[TS][TS + 1];  // inv(L), row-major, padded
   const int t = threadIdx.x;
   const bool pr = pos2dof[J * TS + t] < 0;
-  double row[TS];
+  const unsigned long long padded = __ballot(pr);  // bit c: row and column c are padding
+  {
+    double v[TS];  // all 64 loads in flight before the first use
 #pragma unroll
-  for (int c = 0; c < TS; ++c) {
-    double v = A[t + TS * c];
-    if (pr || pos2dof[J * TS + c] < 0) v = (t == c) ? 1.0 : 0.0;
-    row[c] = v;
-  }
+    for (int c = 0; c < TS; ++c) v[c] = A[t + TS * c];
 #pragma unroll
-  for (int k = 0; k < TS; ++k) {
-    const double d = bcast_lane(row[k], k);
-    if (t == 0 && !(d > 0)) atomicOr(err, 16);
-    const double dk = sqrt(d > 0 ? d : 1.0);
-    row[k] = (t == k) ? dk : row[k] / dk;
-#pragma unroll
-    for (int c = k + 1; c < TS; ++c) row[c] -= row[k] * bcast_lane(row[k], c);
-  }
-#pragma unroll
-  for (int c = 0; c < TS; ++c) {
-    const double v = (t >= c) ? row[c] : 0.0;
-    A[t + TS * c] = v;
-    S[t][c] = v;
+    for (int c = 0; c < TS; ++c) {
+      S[t][c] = (pr || ((padded >> c) & 1ull)) ? ((t == c) ? 1.0 : 0.0) : v[c];
+      X[c][t] = (t == c) ? 1.0 : 0.0;
+    }
   }
   __syncthreads();
-  double x[TS];  // column t of inv(L)
+  for (int kb = 0; kb < TS; kb += PB) {
+    double p[PB];
 #pragma unroll
-  for (int i = 0; i < TS; ++i) {
-    double s = (i == t) ? 1.0 : 0.0;
+    for (int j = 0; j < PB; ++j) p[j] = S[t][kb + j];
 #pragma unroll
-    for (int k = 0; k < i; ++k) s -= S[i][k] * x[k];
-    x[i] = (i >= t) ? s / S[i][i] : 0.0;
+    for (int j = 0; j < PB; ++j) {
+      const double d = bcast_lane(p[j], kb + j);
+      if (t == 0 && !(d > 0)) atomicOr(err, 16);
+      const double dk = sqrt(d > 0 ? d : 1.0);
+      p[j] = (t == kb + j) ? dk : p[j] / dk;
+#pragma unroll
+      for (int jj = j + 1; jj < PB; ++jj) p[jj] -= p[j] * bcast_lane(p[j], kb + jj);
+    }
+#pragma unroll
+    for (int j = 0; j < PB; ++j) S[t][kb + j] = p[j];
+    for (int c = kb + PB; c < TS; ++c) {
+      double v = S[t][c];
+#pragma unroll
+      for (int j = 0; j < PB; ++j) v -= p[j] * bcast_lane(p[j], c);
+      S[t][c] = v;
+    }
   }
+  __syncthreads();
+  for (int kb = 0; kb < TS; kb += PB) {
+    double x[PB];
+#pragma unroll
+    for (int j = 0; j < PB; ++j) x[j] = X[kb + j][t];
+#pragma unroll
+    for (int j = 0; j < PB; ++j) {
+#pragma unroll
+      for (int jj = 0; jj < j; ++jj) x[j] -= S[kb + j][kb + jj] * x[jj];
+      x[j] = (kb + j >= t) ? x[j] / S[kb + j][kb + j] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < PB; ++j) X[kb + j][t] = x[j];
+    for (int i = kb + PB; i < TS; ++i) {
+      double v = X[i][t];
+#pragma unroll
+      for (int j = 0; j < PB; ++j) v -= S[i][kb + j] * x[j];
+      X[i][t] = v;
+    }
+  }
+  __syncthreads();
   double *Di = dinv + (int64_t)J * TT;
   double *DiT = dinvT + (int64_t)J * TT;
-#pragma unroll
-  for (int c = 0; c < TS; ++c) DiT[t + TS * c] = x[c];  // DiT[r + 64c] = X[c][r]
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < TS; ++i) S[i][t] = x[i];
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < TS; ++c) Di[t + TS * c] = S[t][c];
+#pragma unroll 8
+  for (int c = 0; c < TS; ++c) {
+    A[t + TS * c] = (t >= c) ? S[t][c] : 0.0;
+    Di[t + TS * c] = X[t][c];   // Di[r + 64c] = X[r][c]
+    DiT[t + TS * c] = X[c][t];  // DiT[r + 64c] = X[c][r]
+  }
 }
 
 // L_IJ = C_IJ inv(L_JJ)'  and its transposed copy
@@ -439,10 +477,12 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
   __shared__ double sB[TS][TS + 1];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   if (p0 == p1) {  // diagonal tile of the block
-    for (int t = threadIdx.x; t < TT; t += 256) {
-      const int r = t & 63, c = t >> 6;
-      Xo[t] = Di[t];
-      XTo[c + TS * r] = Di[t];
+#pragma unroll
+    for (int q = 0; q < TT / 256; ++q) {
+      const int t = threadIdx.x + 256 * q, r = t & 63, c = t >> 6;
+      const double v = Di[t];
+      Xo[t] = v;
+      XTo[c + TS * r] = v;
     }
     return;
   }
@@ -456,7 +496,14 @@ __global__ __launch_bounds__(256) void k_inv(const int32_t *inv_x, const int32_t
   __syncthreads();
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) sB[4 * tx + j][4 * ty + i] = acc[i][j];  // sB[c][r] = S[r][c]
-  for (int t = threadIdx.x; t < TT; t += 256) sA[t & 63][t >> 6] = Di[t];
+  double vd[TT / 256];
+#pragma unroll
+  for (int q = 0; q < TT / 256; ++q) vd[q] = Di[threadIdx.x + 256 * q];
+#pragma unroll
+  for (int q = 0; q < TT / 256; ++q) {
+    const int t = threadIdx.x + 256 * q;
+    sA[t & 63][t >> 6] = vd[q];
+  }
   __syncthreads();
   double o[4][4] = {};
   for (int k = 0; k < TS; ++k) {

@@ -18,6 +18,8 @@
 // coarse-level matrix values.
 #include "m3_internal.h"
 
+#include <hip/hip_ext.h>
+
 #include <hip/hip_bf16.h>
 
 namespace {
@@ -495,6 +497,17 @@ inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) /
 
 namespace m3 {
 
+thread_local hipEvent_t prof_start = nullptr, prof_stop = nullptr;
+
+// launch, with the profiling events attached when a ProfScope set them
+template <typename... Args, typename F = void (*)(Args...)>
+static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+  if (prof_start)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, prof_start, prof_stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
 // threads per row for a level of n rows: one thread per row when the level
 // alone fills the GPU, four otherwise (on the 43,554-row c1 middle level,
 // 1 / 2 / 4 threads per row measured 2.78 / 2.86 / 2.95 solves/s)
@@ -503,24 +516,19 @@ static int tpr_for(int64_t n) { return n >= 131072 ? 1 : 4; }
 template <class MT, class VT, int MODE, int UNR, bool NT>
 static void launch_smooth_u(hipStream_t s, SellT<MT> K, int64_t n, const VT *x, const VT *r0, const VT *dinv,
                             double w, VT *y, VT *r_out, VT *d_out, double *part) {
-  const int t = tpr_for(n);
-  if (MODE & 8) {  // partial sums are indexed per block of 256 rows
-    hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1, UNR, NT>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
-                       (VT)w, y, r_out, d_out, part);
-    return;
-  }
+  const int t = (MODE & 8) ? 1 : tpr_for(n);  // partial sums are indexed per block of 256 rows
   switch (t) {
     case 2:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 2, UNR, NT>), dim3(nblk(2 * n)), dim3(256), 0, s, K, n, x, r0,
-                         dinv, (VT)w, y, r_out, d_out, part);
+      launch_timed(k_smooth<MT, VT, MODE, 2, UNR, NT>, dim3(nblk(2 * n)), dim3(256), s, K, n, x, r0, dinv, (VT)w,
+                   y, r_out, d_out, part);
       break;
     case 4:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 4, UNR, NT>), dim3(nblk(4 * n)), dim3(256), 0, s, K, n, x, r0,
-                         dinv, (VT)w, y, r_out, d_out, part);
+      launch_timed(k_smooth<MT, VT, MODE, 4, UNR, NT>, dim3(nblk(4 * n)), dim3(256), s, K, n, x, r0, dinv, (VT)w,
+                   y, r_out, d_out, part);
       break;
     default:
-      hipLaunchKernelGGL((k_smooth<MT, VT, MODE, 1, UNR, NT>), dim3(nblk(n)), dim3(256), 0, s, K, n, x, r0, dinv,
-                         (VT)w, y, r_out, d_out, part);
+      launch_timed(k_smooth<MT, VT, MODE, 1, UNR, NT>, dim3(nblk(n)), dim3(256), s, K, n, x, r0, dinv, (VT)w, y,
+                   r_out, d_out, part);
   }
 }
 

@@ -238,20 +238,29 @@ struct m3_ctx {
 };
 
 namespace m3 {
-// bracket a fine-level sweep launch with events when profiling is on
+// Events for the next sweep launch (set by ProfScope): the launch records
+// them itself at the kernel's start and end (hipExtLaunchKernelGGL), so the
+// interval is the kernel's execution even when the host falls behind the GPU.
+extern thread_local hipEvent_t prof_start, prof_stop;
+
+// time the one fine-level sweep launched inside the scope when profiling is on
 struct ProfScope {
   m3_ctx *c;
   int kind;
   hipEvent_t a = nullptr, b = nullptr;
   ProfScope(m3_ctx *c_, int kind_, bool active) : c(c_), kind(kind_) {
     if (!active || !c->prof) return;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, c->stream);
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      if (a) (void)hipEventDestroy(a);
+      a = b = nullptr;
+      return;
+    }
+    prof_start = a;
+    prof_stop = b;
   }
   ~ProfScope() {
     if (!a) return;
-    (void)hipEventRecord(b, c->stream);
+    prof_start = prof_stop = nullptr;
     c->prof_ev[kind].push_back({a, b});
   }
 };
