@@ -351,38 +351,42 @@ __device__ inline double bcast_lane(double v, int lane) {
   return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
-// Cholesky of the diagonal tile and inverse of the factor, one wavefront per
-// tile, in column blocks of PB.  Lane t owns row t.  For block kb the lane
-// keeps its row's PB columns in registers; step j broadcasts L[kb+j][kb+j]
-// and the entries L[c][kb+j] from their owning lanes (v_readlane), then the
-// columns right of the block are updated in LDS, one column per step of a
-// loop.  Every entry receives its updates in ascending column order, the
-// order of the unblocked right-looking elimination, with the same fused
-// multiply-adds; the loops over blocks and columns stay loops, which keeps
-// the kernel small (the fully unrolled elimination was ~220 KB of code, far
-// beyond the instruction cache).  Only the lower triangle is
-// meaningful; the upper part is never read.  The inverse is then formed by
-// blocked forward substitution, lane t computing column t (accumulators in
-// LDS, L read by broadcast), again in ascending column order.
-constexpr int PB = 8;
+// Cholesky of the diagonal tile and inverse of the factor, four wavefronts
+// per tile, in column blocks of PB.  In every wave lane t owns row t.  For
+// block kb each wave keeps its rows' PB columns in registers and factors the
+// panel (step j broadcasts L[kb+j][kb+j] and the entries L[c][kb+j] from their
+// owning lanes with v_readlane; the four waves do the same arithmetic on the
+// same values, so the panel is computed redundantly rather than exchanged);
+// the columns right of the block are then updated in LDS, wave w taking every
+// fourth column.  Every entry receives its updates in ascending column order,
+// the order of the unblocked right-looking elimination, with the same fused
+// multiply-adds.  The loops over blocks and columns stay loops, which keeps
+// the kernel small (a fully unrolled elimination is ~220 KB of code, far
+// beyond the instruction cache).  Only the lower triangle is meaningful; the
+// upper part is never read.  The inverse is then formed by blocked forward
+// substitution, lane t computing column t (accumulators in LDS, L read by
+// broadcast, wave w updating every fourth row below the block), again in
+// ascending column order.
+constexpr int PB = 8, PW = 4;  // panel width, waves per tile
 
-__global__ __launch_bounds__(64) void k_potrf(const int32_t *cols, int col0, const int32_t *diagtile,
-                                              const int32_t *pos2dof, double *tiles, double *dinv,
-                                              double *dinvT, int *err) {
+__global__ __launch_bounds__(64 * PW) void k_potrf(const int32_t *cols, int col0, const int32_t *diagtile,
+                                                   const int32_t *pos2dof, double *tiles, double *dinv,
+                                                   double *dinvT, int *err) {
   const int J = cols[col0 + blockIdx.x];
   double *A = tiles + (int64_t)diagtile[J] * TT;
   __shared__ double S[TS][TS + 1];  // L, row-major, padded
   __shared__ double X[TS][TS + 1];  // inv(L), row-major, padded
-  const int t = threadIdx.x;
+  const int t = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool pr = pos2dof[J * TS + t] < 0;
   const unsigned long long padded = __ballot(pr);  // bit c: row and column c are padding
   {
-    double v[TS];  // all 64 loads in flight before the first use
+    double v[TS / PW];  // this thread's 16 loads in flight before the first use
 #pragma unroll
-    for (int c = 0; c < TS; ++c) v[c] = A[t + TS * c];
+    for (int q = 0; q < TS / PW; ++q) v[q] = A[t + TS * (w + PW * q)];
 #pragma unroll
-    for (int c = 0; c < TS; ++c) {
-      S[t][c] = (pr || ((padded >> c) & 1ull)) ? ((t == c) ? 1.0 : 0.0) : v[c];
+    for (int q = 0; q < TS / PW; ++q) {
+      const int c = w + PW * q;
+      S[t][c] = (pr || ((padded >> c) & 1ull)) ? ((t == c) ? 1.0 : 0.0) : v[q];
       X[c][t] = (t == c) ? 1.0 : 0.0;
     }
   }
@@ -391,49 +395,55 @@ __global__ __launch_bounds__(64) void k_potrf(const int32_t *cols, int col0, con
     double p[PB];
 #pragma unroll
     for (int j = 0; j < PB; ++j) p[j] = S[t][kb + j];
+    __syncthreads();  // every wave has its panel copy before wave 0 overwrites it
 #pragma unroll
     for (int j = 0; j < PB; ++j) {
       const double d = bcast_lane(p[j], kb + j);
-      if (t == 0 && !(d > 0)) atomicOr(err, 16);
+      if (threadIdx.x == 0 && !(d > 0)) atomicOr(err, 16);
       const double dk = sqrt(d > 0 ? d : 1.0);
       p[j] = (t == kb + j) ? dk : p[j] / dk;
 #pragma unroll
       for (int jj = j + 1; jj < PB; ++jj) p[jj] -= p[j] * bcast_lane(p[j], kb + jj);
     }
+    if (w == 0) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j) S[t][kb + j] = p[j];
-    for (int c = kb + PB; c < TS; ++c) {
+      for (int j = 0; j < PB; ++j) S[t][kb + j] = p[j];
+    }
+    for (int c = kb + PB + w; c < TS; c += PW) {
       double v = S[t][c];
 #pragma unroll
       for (int j = 0; j < PB; ++j) v -= p[j] * bcast_lane(p[j], c);
       S[t][c] = v;
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int kb = 0; kb < TS; kb += PB) {
     double x[PB];
 #pragma unroll
     for (int j = 0; j < PB; ++j) x[j] = X[kb + j][t];
+    __syncthreads();  // every wave has its copy before wave 0 overwrites it
 #pragma unroll
     for (int j = 0; j < PB; ++j) {
 #pragma unroll
       for (int jj = 0; jj < j; ++jj) x[j] -= S[kb + j][kb + jj] * x[jj];
       x[j] = (kb + j >= t) ? x[j] / S[kb + j][kb + j] : 0.0;
     }
+    if (w == 0) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j) X[kb + j][t] = x[j];
-    for (int i = kb + PB; i < TS; ++i) {
+      for (int j = 0; j < PB; ++j) X[kb + j][t] = x[j];
+    }
+    for (int i = kb + PB + w; i < TS; i += PW) {
       double v = X[i][t];
 #pragma unroll
       for (int j = 0; j < PB; ++j) v -= S[i][kb + j] * x[j];
       X[i][t] = v;
     }
+    __syncthreads();
   }
-  __syncthreads();
   double *Di = dinv + (int64_t)J * TT;
   double *DiT = dinvT + (int64_t)J * TT;
-#pragma unroll 8
-  for (int c = 0; c < TS; ++c) {
+#pragma unroll 4
+  for (int c = w; c < TS; c += PW) {
     A[t + TS * c] = (t >= c) ? S[t][c] : 0.0;
     Di[t + TS * c] = X[t][c];   // Di[r + 64c] = X[r][c]
     DiT[t + TS * c] = X[c][t];  // DiT[r + 64c] = X[c][r]
@@ -937,7 +947,7 @@ void chol_factor(m3_ctx *c) {
       hipLaunchKernelGGL(k_update, dim3(u1 - u0), dim3(256), 0, s, ch.upd_tile.p, ch.upd_pair_ptr.p,
                          ch.upd_pair_end.p, ch.upd_pairs.p, u0, ch.tiles.p);
     const int c0 = ch.h_lev_ptr[l], c1 = ch.h_lev_ptr[l + 1];
-    hipLaunchKernelGGL(k_potrf, dim3(c1 - c0), dim3(64), 0, s, ch.lev_cols.p, c0, ch.diagtile.p,
+    hipLaunchKernelGGL(k_potrf, dim3(c1 - c0), dim3(64 * PW), 0, s, ch.lev_cols.p, c0, ch.diagtile.p,
                        ch.pos2dof.p, ch.tiles.p, ch.dinv.p, ch.dinvT.p, c->errflag.p);
     const int t0 = ch.h_trsm_ptr[l], t1 = ch.h_trsm_ptr[l + 1];
     if (t1 > t0)
