@@ -180,8 +180,7 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
   __shared__ double s_J[NIP][9];      // rows: original-frame x_i; cols: d xi_k
   __shared__ double s_wv[NIP], s_w[NIP], s_wd[NIP], s_wf[NIP], s_rh[NIP][3];
   __shared__ double s_g[NIP][10][3];  // dN_n/dx_i at each IP (original frame)
-  __shared__ double s_K[30][31];
-  __shared__ double s_Kr[30][31];
+  __shared__ double s_K[30][31];  // original frame, then rotated in place
   __shared__ double s_RR[10][9];
   __shared__ double s_Gr[30][4];
   __shared__ double s_F[30];
@@ -200,58 +199,46 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
   // ---------------- per-IP geometry (one lane per integration point)
   if (lane < NIP) {
     const int ip = lane;
-    double th[20], ph[20], rr[20];
-    int nn = 4;
+    // J_SL[k][m] = sum_n dN_n/dxi_k * X_m(n) and the IP's interpolated
+    // coordinates, accumulated node by node as each node's spherical
+    // coordinates are formed (no per-node arrays: fewer registers, more
+    // resident waves)
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    double thi = 0, phi = 0, ri = 0;
+    auto add_node = [&](double th, double ph, double rr, const double *dn, double N) {
+      for (int k = 0; k < 3; ++k) {
+        J[k][0] += dn[k] * th;
+        J[k][1] += dn[k] * ph;
+        J[k][2] += dn[k] * rr;
+      }
+      thi += N * th;
+      phi += N * ph;
+      ri += N * rr;
+    };
     if (cls == 3) {
-      nn = 20;
       const double *x20 = a.X20 + (int64_t)a.iso_idx[e] * 60;
       for (int n = 0; n < 20; ++n) {
+        double th, ph, rr;
         // 90-degree rotation about X: x'=x, y'=-z, z'=y
-        cart2sph(x20[3 * n], -x20[3 * n + 2], x20[3 * n + 1], th[n], ph[n], rr[n]);
+        cart2sph(x20[3 * n], -x20[3 * n + 2], x20[3 * n + 1], th, ph, rr);
+        add_node(th, ph, rr, c_dN20[ip][n], c_N20[ip][n]);
       }
     } else if (cls == 2) {
       for (int n = 0; n < 4; ++n) {
         const double *x = a.X + 3 * (int64_t)s_nod[n];
-        cart2sph(x[0], -x[2], x[1], th[n], ph[n], rr[n]);
+        double th, ph, rr;
+        cart2sph(x[0], -x[2], x[1], th, ph, rr);
+        add_node(th, ph, rr, c_dN4[n], c_N4[ip][n]);
       }
     } else {
       for (int n = 0; n < 4; ++n) {
-        const double *s = a.S + 3 * (int64_t)s_nod[n];
-        th[n] = s[0];
-        ph[n] = s[1];
-        rr[n] = s[2];
+        const double *sv = a.S + 3 * (int64_t)s_nod[n];
+        double ph = sv[1];
         if (cls == 1) {  // 180-degree rotation about Z
-          ph[n] += M_PI;
-          if (ph[n] > 2.0 * M_PI) ph[n] -= 2.0 * M_PI;
+          ph += M_PI;
+          if (ph > 2.0 * M_PI) ph -= 2.0 * M_PI;
         }
-      }
-    }
-    // J_SL[k][m] = sum_n dN_n/dxi_k * X_m(n)
-    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    double thi = 0, phi = 0, ri = 0;
-    if (nn == 20) {
-      for (int n = 0; n < 20; ++n) {
-        for (int k = 0; k < 3; ++k) {
-          const double dn = c_dN20[ip][n][k];
-          J[k][0] += dn * th[n];
-          J[k][1] += dn * ph[n];
-          J[k][2] += dn * rr[n];
-        }
-        thi += c_N20[ip][n] * th[n];
-        phi += c_N20[ip][n] * ph[n];
-        ri += c_N20[ip][n] * rr[n];
-      }
-    } else {
-      for (int n = 0; n < 4; ++n) {
-        for (int k = 0; k < 3; ++k) {
-          const double dn = c_dN4[n][k];
-          J[k][0] += dn * th[n];
-          J[k][1] += dn * ph[n];
-          J[k][2] += dn * rr[n];
-        }
-        thi += c_N4[ip][n] * th[n];
-        phi += c_N4[ip][n] * ph[n];
-        ri += c_N4[ip][n] * rr[n];
+        add_node(sv[0], ph, sv[2], c_dN4[n], c_N4[ip][n]);
       }
     }
     const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
@@ -377,40 +364,44 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
   }
   __syncthreads();
   if (!a.write_global) return;
-  // ---------------- rotate blocks to the nodal (theta, phi, r) frames
+  // ---------------- rotate blocks to the nodal (theta, phi, r) frames, in
+  // place: each 3x3 block is read, rotated and written by one lane (one LDS
+  // copy of the 30x30 matrix instead of two keeps more waves resident)
   for (int t = lane; t < 100; t += 64) {
     const int na = t / 10, nb = t % 10;
-    double T1[3][3];
+    double Kb[3][3], T1[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) Kb[i][j] = s_K[3 * na + i][3 * nb + j];
     for (int i = 0; i < 3; ++i)
       for (int dd = 0; dd < 3; ++dd) {
         double acc = 0;
-        for (int j = 0; j < 3; ++j) acc += s_K[3 * na + i][3 * nb + j] * s_RR[nb][3 * j + dd];
+        for (int j = 0; j < 3; ++j) acc += Kb[i][j] * s_RR[nb][3 * j + dd];
         T1[i][dd] = acc;
       }
     for (int cc = 0; cc < 3; ++cc)
       for (int dd = 0; dd < 3; ++dd) {
         double acc = 0;
         for (int i = 0; i < 3; ++i) acc += s_RR[na][3 * i + cc] * T1[i][dd];
-        s_Kr[3 * na + cc][3 * nb + dd] = acc;
+        s_K[3 * na + cc][3 * nb + dd] = acc;
       }
   }
   __syncthreads();
   // ---------------- BC lift (bc_rhs = K U0) and removal of fixed dofs
   if (lane < 30) {
     double acc = 0;
-    for (int q = 0; q < 30; ++q) acc += s_Kr[lane][q] * s_u0[q];
+    for (int q = 0; q < 30; ++q) acc += s_K[lane][q] * s_u0[q];
     if (acc != 0.0) atomicAdd(&a.bc_rhs[3 * (int64_t)s_nod[lane / 3] + lane % 3], acc);
   }
   __syncthreads();
   for (int t = lane; t < 900; t += 64) {
     const int p = t / 30, q = t % 30;
-    s_Kr[p][q] *= s_msk[p] * s_msk[q];
+    s_K[p][q] *= s_msk[p] * s_msk[q];
   }
   __syncthreads();
   if (a.Kel) {
     for (int t = lane; t < 465; t += 64) {
       const int pq = c_pq[t];
-      a.Kel[e * 465 + t] = s_Kr[pq & 255][pq >> 8];
+      a.Kel[e * 465 + t] = s_K[pq & 255][pq >> 8];
     }
   }
   // ---------------- scatter K blocks (unless the row-owner gather k_gatherK sums them)
@@ -420,7 +411,7 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
     const int64_t pos = slot[t];
     for (int cc = 0; cc < 3; ++cc)
       for (int dd = 0; dd < 3; ++dd)
-        atomicAdd(a.K.val + (3 * cc + dd) * a.K.nslot + pos, s_Kr[3 * na + cc][3 * nb + dd]);
+        atomicAdd(a.K.val + (3 * cc + dd) * a.K.nslot + pos, s_K[3 * na + cc][3 * nb + dd]);
   }
   // ---------------- G (rotated): G_rot[a,:,v] = RR_a' G[a,:,v]
   if (lane < 40) {
