@@ -318,22 +318,39 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
   __syncthreads();
   // ---------------- K (original frame): K[(a,c)][(b,d)] =
   //   sum_ip w eta [delta_cd g_a.g_b + g_b[c] g_a[d] + C2 g_b[d] g_a[c]]
+  //   one lane per 3x3 node block (na >= nb, 55 blocks): the two gradients of
+  //   an integration point are read once for the block's nine entries
   const double C2 = a.C2;
-  for (int t = lane; t < 465; t += 64) {
-    const int pq = c_pq[t];
-    const int p = pq & 255, q = pq >> 8;
-    const int na = p / 3, c = p % 3, nb = q / 3, d = q % 3;
-    double acc = 0;
-    for (int ip = 0; ip < NIP; ++ip) {
-      const double *ga = s_g[ip][na];
-      const double *gb = s_g[ip][nb];
-      double v = gb[c] * ga[d] + C2 * gb[d] * ga[c];
-      if (c == d) v += ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-      acc += s_wv[ip] * v;
+  if (lane < 55) {
+    int na = 0, nb = lane;
+    while (nb > na) {
+      nb -= na + 1;
+      ++na;
     }
-    s_K[p][q] = acc;
-    s_K[q][p] = acc;
-    if (a.dbgK) a.dbgK[e * 465 + t] = acc;
+    double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+    for (int ip = 0; ip < NIP; ++ip) {
+      const double ga[3] = {s_g[ip][na][0], s_g[ip][na][1], s_g[ip][na][2]};
+      const double gb[3] = {s_g[ip][nb][0], s_g[ip][nb][1], s_g[ip][nb][2]};
+      const double wv = s_wv[ip];
+      const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double v = gb[c] * ga[d] + C2 * gb[d] * ga[c];
+          if (c == d) v += dot;
+          acc[c][d] += wv * v;
+        }
+    }
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) {
+        if (na == nb && c < d) continue;  // upper half of a diagonal block: its mirror is written
+        const int p = 3 * na + c, q = 3 * nb + d;  // p >= q
+        s_K[p][q] = acc[c][d];
+        s_K[q][p] = acc[c][d];
+        if (a.dbgK) a.dbgK[e * 465 + (30 * q - q * (q - 1) / 2 + (p - q))] = acc[c][d];
+      }
   }
   // ---------------- G (original frame), M, Fb
   for (int t = lane; t < 120; t += 64) {
