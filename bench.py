@@ -220,6 +220,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--visc-contrast", type=float, default=None,
                     help="default per config: c0 1e3, c1 1e4, c2 1e5, c4 1e6")
+    ap.add_argument("--rtol-U-first", type=float, default=0.0,
+                    help="initial velocity solve of the warm-started legs: 0 keeps the reference's "
+                         "tolerance relative to the starting residual; > 0 stops at this fraction of |f_U|")
     ap.add_argument("--path", default="stokes", choices=["stokes", "diffusion"])
     ap.add_argument("--grid", default="vertex", choices=["vertex", "p2split"])
     args = ap.parse_args()
@@ -286,7 +289,7 @@ def main():
 
     def warm(form, U0, P0):
         return ctx.flow3d(Visc, Dens, phys, P0=P0, U0=U0, precision=form, rtol_U_final=0.5 * args.tol,
-                          tol_Pat_abs=tol_abs)
+                          tol_Pat_abs=tol_abs, rtol_U_first=args.rtol_U_first)
 
     def extrap(form):
         _, U0, P0 = net.extrapolate(t_k, order)

@@ -127,6 +127,12 @@ typedef struct {
                           > 0: absolute RMS pressure-residual tolerance, so that
                           runs from different starting points stop at the same
                           residual */
+  double rtol_U_first; /* 0: reference behaviour (the initial velocity solve stops
+                          at rtol_U * rtol_restart relative to its own starting
+                          residual, flow3d_patera_mg_sph_p.m:413-423).  > 0: it
+                          stops once |r_U| <= rtol_U_first |f_U| (RMS over the
+                          free dofs), so a better starting point needs fewer
+                          iterations instead of the same relative reduction */
 } m3_patera_opts;
 
 typedef struct {

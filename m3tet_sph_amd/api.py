@@ -66,7 +66,8 @@ class PateraOpts(C.Structure):
     _fields_ = [("rtol_Pat", C.c_double), ("rtol_U", C.c_double), ("itmax_U", C.c_int32),
                 ("itmin_Pat", C.c_int32), ("itmax_Pat", C.c_int32), ("pc_Pat", C.c_int32),
                 ("wght_MM", C.c_double), ("precision", C.c_int32), ("warm_U", C.c_int32),
-                ("rtol_U_final", C.c_double), ("itmax_U_final", C.c_int32), ("tol_Pat_abs", C.c_double)]
+                ("rtol_U_final", C.c_double), ("itmax_U_final", C.c_int32), ("tol_Pat_abs", C.c_double),
+                ("rtol_U_first", C.c_double)]
 
 
 class Info(C.Structure):
@@ -210,10 +211,11 @@ def phys_struct(phys, nip=14, type_Dmat="23rd"):
 
 def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=120,
                 wght_MM=1.0, precision="fp64", warm_U=False, rtol_U_final=0.0, itmax_U_final=60,
-                tol_Pat_abs=0.0):
+                tol_Pat_abs=0.0, rtol_U_first=0.0):
     """Defaults of flow3d_patera_mg_sph_p.m:70-98 (rtol_U_final=0 keeps the
     reference's final velocity tolerance, tol_Pat_abs=0 its relative
-    pressure tolerance)."""
+    pressure tolerance, rtol_U_first=0 its relative initial velocity
+    tolerance)."""
     o = PateraOpts()
     o.rtol_Pat, o.rtol_U, o.itmax_U = rtol_Pat, rtol_U, itmax_U
     o.itmin_Pat, o.itmax_Pat, o.pc_Pat, o.wght_MM = itmin_Pat, itmax_Pat, 3, wght_MM
@@ -221,6 +223,7 @@ def patera_opts(rtol_Pat=1e-4, rtol_U=5e-3, itmax_U=20, itmin_Pat=3, itmax_Pat=1
     o.warm_U = int(warm_U)
     o.rtol_U_final, o.itmax_U_final = rtol_U_final, itmax_U_final
     o.tol_Pat_abs = tol_Pat_abs
+    o.rtol_U_first = rtol_U_first
     return o
 
 

@@ -639,7 +639,13 @@ static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, 
     return n;
   };
   rhs_of_P();
-  int nitU = vel_solve(Uf, Rhs, rtol_U * rtol_restart, false);
+  int nitU = 0;
+  if (o->rtol_U_first > 0) {  // fixed-denominator initial velocity tolerance
+    const double normf = std::sqrt(m3::dot(c, Rhs, Rhs, n3) / (double)f.nfree);
+    nitU = vel_solve(Uf, Rhs, o->rtol_U_first * normf, true);
+  } else {
+    nitU = vel_solve(Uf, Rhs, rtol_U * rtol_restart, false);
+  }
   pres_res(true);
   double norm_rP = normdf(r, nV, (double)nV);
   const double norm_rP_init = norm_rP;

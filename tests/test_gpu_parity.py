@@ -275,6 +275,29 @@ def test_c1_mixed_precision_reaches_tolerance(c1ctx):
     assert eu < 1e-4 and ep < 1e-4
 
 
+def test_c1_fixed_initial_velocity_tolerance(c1ctx):
+    """rtol_U_first: started from a perturbed converged state, the initial
+    velocity solve stops at a fraction of |f_U| instead of a fraction of its
+    own starting residual; the result meets the same fixed-denominator
+    residual with fewer velocity iterations."""
+    from m3tet_sph_amd import api
+    m, inp, ctx = c1ctx
+    U, P, i0 = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], rtol_Pat=1e-7, rtol_U_final=1e-7)
+    rng = np.random.default_rng(5)
+    U0 = U * (1 + 1e-4 * rng.standard_normal(len(U)))
+    P0 = P * (1 + 1e-4 * rng.standard_normal(len(P)))
+    tol_abs = 0.8e-6 * i0["norm_rP0"]
+    out = {}
+    for first in (0.0, 4e-7):
+        U1, P1, info = ctx.flow3d(inp["Visc"], inp["Dens"], inp["phys"], U0=U0, P0=P0, rtol_U_final=5e-7,
+                                  tol_Pat_abs=tol_abs, rtol_U_first=first)
+        rU, fU, rP = api.residual(ctx, U1, P1)
+        log("rtol_U_first", first, info, rU / fU, rP / i0["norm_rP0"])
+        assert rU / fU < 1e-6 and rP / i0["norm_rP0"] < 1e-6
+        out[first] = info["nitU"] + info["nitZ"]
+    assert out[4e-7] < out[0.0], out
+
+
 def test_reference_named_restriction_and_cholesky(c0, c0_inputs, ora):
     """restrict_stiffmat / intermesh_transfer_matrix / cholesky_factorization /
     cholesky_solve with the reference's argument meaning, on a context bound
