@@ -29,6 +29,9 @@ Forms:
         extrapolation plus the network's correction, computed from
         (T, ln eta, the extrapolated state, T minus its extrapolation),
         followed by the solve;
+    (the extrapolated and learned starts are formed on the GPU and handed to
+    the solver there, m3_flow3d_patera_sph_dstart; the previous-step start
+    comes from host memory, as the reference passes VAR to the solver)
     all stopping at the same fixed residual.  ``value`` is the mixed+learned
     rate; the others are reported beside it.
   fp64 / mixed: cold solves of the seed-0 temperature field.
@@ -291,13 +294,17 @@ def main():
         return ctx.flow3d(Visc, Dens, phys, P0=P0, U0=U0, precision=form, rtol_U_final=0.5 * args.tol,
                           tol_Pat_abs=tol_abs, rtol_U_first=args.rtol_U_first)
 
+    def from_device(form):  # the start that the network context holds on the GPU
+        return ctx.flow3d(Visc, Dens, phys, start_device=net.start_device(), precision=form,
+                          rtol_U_final=0.5 * args.tol, tol_Pat_abs=tol_abs, rtol_U_first=args.rtol_U_first)
+
     def extrap(form):
-        _, U0, P0 = net.extrapolate(t_k, order)
-        return warm(form, U0, P0)
+        net.extrapolate(t_k, order, device=True)
+        return from_device(form)
 
     def learned(form):
-        U0, P0 = net.warm_start_hist(t_k, order, Tv, Visc)
-        return warm(form, U0, P0)
+        net.warm_start_hist(t_k, order, Tv, Visc, device=True)
+        return from_device(form)
 
     def rel_res(U, P):
         rU, fU, rP = api.residual(ctx, U, P)

@@ -129,12 +129,18 @@ int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const do
 #define M3_GNN_HIST 4
 int m3_gnn_clear_history(m3_gnn *g);
 int m3_gnn_push_state(m3_gnn *g, double t, const double *T, const double *U, const double *P);
-/* Base state at time t; any output may be NULL. */
+/* Base state at time t; any output may be NULL.  With all three NULL the
+ * base state is kept on the device as the start (m3_gnn_start_device). */
 int m3_gnn_extrapolate(m3_gnn *g, double t, int32_t order, double *T_base, double *U_base, double *P_base);
 int m3_gnn_add_sample_hist(m3_gnn *g, int32_t idx, double t, int32_t order, const double *T,
                            const double *Visc, const double *U_next, const double *P_next);
 int m3_gnn_warm_start_hist(m3_gnn *g, double t, int32_t order, const double *T, const double *Visc,
                            double *U0, double *P0);
+/* With U0 = P0 = NULL the warm-start entry points leave the start on the
+ * device only; these are its device addresses (3*nnod and nvert values,
+ * valid until the next warm start or m3_gnn_destroy), ready when the call
+ * that produced them returned.  Pass them to m3_flow3d_patera_sph_dstart. */
+int m3_gnn_start_device(m3_gnn *g, const double **U0_dev, const double **P0_dev);
 
 #ifdef __cplusplus
 }

@@ -206,6 +206,15 @@ int m3_flow3d_patera_sph(m3_ctx *ctx, const double *Visc, const double *Dens,
                          const m3_phys *phys, double *U, double *P,
                          const m3_patera_opts *opts, m3_info *info);
 
+/* The same solve with the starting point already in device memory on the
+ * context's GPU (for instance the warm start that m3_gnn_warm_start_hist left
+ * there, m3_gnn_start_device): P0_dev (nVnod) and, with opts->warm_U = 1,
+ * U0_dev (3*nnod, rotated frame) are copied device to device, so the start
+ * makes no round trip through host memory.  U and P are outputs only. */
+int m3_flow3d_patera_sph_dstart(m3_ctx *ctx, const double *Visc, const double *Dens,
+                                const m3_phys *phys, const double *U0_dev, const double *P0_dev,
+                                double *U, double *P, const m3_patera_opts *opts, m3_info *info);
+
 /* Operator application for parity checks.  which: 0 = K on level `level`
  * (level 0 = K1), 1 = GG (nVnod -> 3*nnod), 2 = GG' (3*nnod -> nVnod),
  * 3 = MM (full symmetric), 4 = prolongation level+1 -> level (masked),

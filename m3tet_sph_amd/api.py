@@ -97,6 +97,8 @@ LIB.m3_mgpcg.argtypes = [C.c_void_p, D, D, C.c_double, C.c_int32, C.c_int32, C.c
 LIB.m3_patera.argtypes = [C.c_void_p, D, D, C.POINTER(PateraOpts), C.POINTER(Info)]
 LIB.m3_flow3d_patera_sph.argtypes = [C.c_void_p, D, D, C.POINTER(Phys), D, D,
                                      C.POINTER(PateraOpts), C.POINTER(Info)]
+LIB.m3_flow3d_patera_sph_dstart.argtypes = [C.c_void_p, D, D, C.POINTER(Phys), C.c_void_p, C.c_void_p, D, D,
+                                            C.POINTER(PateraOpts), C.POINTER(Info)]
 LIB.m3_apply.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D, D]
 LIB.m3_get_vector.argtypes = [C.c_void_p, C.c_int32, C.c_int32, D]
 LIB.m3_stats.argtypes = [C.c_void_p, I64, C.c_int32]
@@ -178,8 +180,8 @@ LIB.m3_rock_prop.restype = C.c_int
 LIB.m3_calculate_dt.restype = C.c_int
 for _f in ("m3_create", "m3_set_mesh", "m3_set_bc", "m3_assemble", "m3_element_matrices",
            "m3_restrict_stiffmat", "m3_cholesky_factorization", "m3_cholesky_solve", "m3_mgpcg",
-           "m3_patera", "m3_flow3d_patera_sph", "m3_apply", "m3_get_vector", "m3_stats",
-           "m3_sl_advect"):
+           "m3_patera", "m3_flow3d_patera_sph", "m3_flow3d_patera_sph_dstart", "m3_apply", "m3_get_vector",
+           "m3_stats", "m3_sl_advect"):
     getattr(LIB, _f).restype = C.c_int
 
 
@@ -386,16 +388,24 @@ class StokesContext:
         self._check(LIB.m3_patera(self._h, _p(U), _p(P), C.byref(o), C.byref(info)))
         return U, P, info.as_dict()
 
-    def flow3d(self, Visc, Dens, phys, P0=None, U0=None, type_Dmat="23rd", **opts):
+    def flow3d(self, Visc, Dens, phys, P0=None, U0=None, type_Dmat="23rd", start_device=None, **opts):
+        """start_device=(U0_dev, P0_dev): device addresses of the starting point
+        (WarmStartNet.start_device()), used instead of U0 / P0."""
         n3 = 3 * self.mesh.nnod
-        U = np.zeros(n3) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).copy()
-        P = np.zeros(self.mesh.nVnod) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
         V, R = _fields(Visc, Dens)
         info = Info()
+        ph = C.byref(phys_struct(phys, type_Dmat=type_Dmat))
+        if start_device is not None:
+            U, P = np.empty(n3), np.empty(self.mesh.nVnod)
+            o = patera_opts(warm_U=True, **opts)
+            self._check(LIB.m3_flow3d_patera_sph_dstart(self._h, V, R, ph, C.c_void_p(start_device[0]),
+                                                        C.c_void_p(start_device[1]), _p(U), _p(P), C.byref(o),
+                                                        C.byref(info)))
+            return U, P, info.as_dict()
+        U = np.zeros(n3) if U0 is None else np.ascontiguousarray(U0, dtype=np.float64).copy()
+        P = np.zeros(self.mesh.nVnod) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
         o = patera_opts(warm_U=U0 is not None, **opts)
-        self._check(LIB.m3_flow3d_patera_sph(self._h, V, R,
-                                             C.byref(phys_struct(phys, type_Dmat=type_Dmat)),
-                                             _p(U), _p(P), C.byref(o), C.byref(info)))
+        self._check(LIB.m3_flow3d_patera_sph(self._h, V, R, ph, _p(U), _p(P), C.byref(o), C.byref(info)))
         return U, P, info.as_dict()
 
     def rock_prop(self, T, params, download=True):

@@ -449,6 +449,7 @@ struct m3_gnn {
   DBuf<double> hT[NH], hU[NH], hP[NH];
   double ht[NH] = {};
   int nhist = 0, head = 0;
+  bool have_start = false;  // U0, P0 hold a warm start
 };
 
 namespace {
@@ -594,7 +595,8 @@ void make_base(m3_gnn *g, double t, int order, bool scales = true) {
 }
 
 // U0 = Up + network change (vertices; edge nodes take the mean of their two
-// vertices' change), P0 = Pp + change; downloaded to the caller
+// vertices' change), P0 = Pp + change; downloaded to the caller unless both
+// outputs are NULL (the start then stays in g->U0, g->P0)
 void apply_and_download(m3_gnn *g, double *U0, double *P0) {
   hipStream_t s = g->stream;
   const int64_t n = g->n, nn = g->nnod;
@@ -604,9 +606,10 @@ void apply_and_download(m3_gnn *g, double *U0, double *P0) {
     hipLaunchKernelGGL(k_apply_edge, dim3(nblk(nn - n)), dim3(256), 0, s, n, nn - n, g->par.p, g->dU.p, g->Up.p,
                        g->U0.p);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemcpyAsync(U0, g->U0.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipMemcpyAsync(P0, g->P0.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (U0) HIPCHECK(hipMemcpyAsync(U0, g->U0.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
+  if (P0) HIPCHECK(hipMemcpyAsync(P0, g->P0.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
+  g->have_start = true;
 }
 
 }  // namespace
@@ -895,7 +898,7 @@ int m3_gnn_warm_start(m3_gnn *g, const double *T, const double *T_prev, const do
                       const double *P_prev, double *U0, double *P0) {
   return gguard(g, [&] {
     require_params(g);
-    M3_REQUIRE(U0 && P0, M3_E_ARG, "null argument");
+    M3_REQUIRE((U0 == nullptr) == (P0 == nullptr), M3_E_ARG, "U0 and P0 must both be given or both be NULL");
     upload_state(g, T, T_prev, Visc, U_prev, P_prev);
     features_to(g, g->X.p);
     forward(g, g->X.p);
@@ -938,6 +941,13 @@ int m3_gnn_extrapolate(m3_gnn *g, double t, int32_t order, double *T_base, doubl
     hipStream_t s = g->stream;
     const int64_t n = g->n, nn = g->nnod;
     HIPCHECK(hipGetLastError());
+    if (!T_base && !U_base && !P_base) {  // the base state becomes the device start
+      HIPCHECK(hipMemcpyAsync(g->U0.p, g->Up.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(g->P0.p, g->Pp.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      g->have_start = true;
+      return;
+    }
     if (T_base) HIPCHECK(hipMemcpyAsync(T_base, g->Tp.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
     if (U_base) HIPCHECK(hipMemcpyAsync(U_base, g->Up.p, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, s));
     if (P_base) HIPCHECK(hipMemcpyAsync(P_base, g->Pp.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -968,12 +978,21 @@ int m3_gnn_warm_start_hist(m3_gnn *g, double t, int32_t order, const double *T, 
                            double *P0) {
   return gguard(g, [&] {
     require_params(g);
-    M3_REQUIRE(U0 && P0, M3_E_ARG, "null argument");
+    M3_REQUIRE((U0 == nullptr) == (P0 == nullptr), M3_E_ARG, "U0 and P0 must both be given or both be NULL");
     upload_new(g, T, Visc);
     make_base(g, t, order);
     features_to(g, g->X.p);
     forward(g, g->X.p);
     apply_and_download(g, U0, P0);
+  });
+}
+
+int m3_gnn_start_device(m3_gnn *g, const double **U0_dev, const double **P0_dev) {
+  return gguard(g, [&] {
+    M3_REQUIRE(U0_dev && P0_dev, M3_E_ARG, "null argument");
+    M3_REQUIRE(g->have_start, M3_E_STATE, "no warm start computed yet");
+    *U0_dev = g->U0.p;
+    *P0_dev = g->P0.p;
   });
 }
 

@@ -589,7 +589,8 @@ int m3_mgpcg(m3_ctx *c, const double *b, double *x, double tol, int32_t is_abs_t
 }
 
 // ---------------------------------------------------------------- Patera
-static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, m3_info *info) {
+static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, m3_info *info,
+                      const double *U0d = nullptr, const double *P0d = nullptr) {
   M3_REQUIRE(c->factored, M3_E_STATE, "operators are not ready (assemble, restrict, factorise)");
   M3_REQUIRE(o && o->pc_Pat == 3, M3_E_ARG, "only pc_Pat = 3 is supported");
   hipStream_t s = c->stream;
@@ -613,9 +614,15 @@ static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, 
   m3::axpy(s, -1.0, c->bc_rhs.p, fbbc.p, n3);
   m3::scale_mask(s, fbbc.p, f.mask.p, fbbc.p, n3);
   m3::spmv_GT(s, c->GT, c->U0.p, gU0.p);
-  HIPCHECK(hipMemcpyAsync(Pd, P, sizeof(double) * nV, hipMemcpyHostToDevice, s));
+  if (P0d)  // starting point already on the device
+    HIPCHECK(hipMemcpyAsync(Pd, P0d, sizeof(double) * nV, hipMemcpyDeviceToDevice, s));
+  else
+    HIPCHECK(hipMemcpyAsync(Pd, P, sizeof(double) * nV, hipMemcpyHostToDevice, s));
   if (o->warm_U) {
-    HIPCHECK(hipMemcpyAsync(Uf, U, sizeof(double) * n3, hipMemcpyHostToDevice, s));
+    if (U0d)
+      HIPCHECK(hipMemcpyAsync(Uf, U0d, sizeof(double) * n3, hipMemcpyDeviceToDevice, s));
+    else
+      HIPCHECK(hipMemcpyAsync(Uf, U, sizeof(double) * n3, hipMemcpyHostToDevice, s));
     m3::scale_mask(s, Uf, f.mask.p, Uf, n3);
   } else {
     Uf = c->Uv.p;
@@ -782,6 +789,42 @@ int m3_flow3d_patera_sph(m3_ctx *c, const double *Visc, const double *Dens, cons
     do_factor(c);
     const double t_fa = tf.stop();
     do_patera(c, U, P, opts, info);
+    if (info) {
+      info->t_assemble = t_as;
+      info->t_restrict = t_re;
+      info->t_factor = t_fa;
+      info->t_total = secs(t0, clk::now());
+    }
+  });
+}
+
+// p must be device memory on the context's GPU
+static void require_device_ptr(const m3_ctx *c, const void *p, const char *what) {
+  M3_REQUIRE(p, M3_E_ARG, what);
+  hipPointerAttribute_t a{};
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) (void)hipGetLastError();  // not a HIP allocation: clear the error state
+  M3_REQUIRE(e == hipSuccess && a.type == hipMemoryTypeDevice && a.device == c->device, M3_E_ARG, what);
+}
+
+int m3_flow3d_patera_sph_dstart(m3_ctx *c, const double *Visc, const double *Dens, const m3_phys *phys,
+                                const double *U0_dev, const double *P0_dev, double *U, double *P,
+                                const m3_patera_opts *opts, m3_info *info) {
+  return guard(c, [&] {
+    M3_REQUIRE(U && P && opts, M3_E_ARG, "null argument");
+    require_device_ptr(c, P0_dev, "P0_dev must be device memory on the context's GPU");
+    if (opts->warm_U) require_device_ptr(c, U0_dev, "U0_dev must be device memory on the context's GPU");
+    auto t0 = clk::now();
+    Timer ta(c->stream);
+    do_assemble(c, Visc, Dens, phys);
+    const double t_as = ta.stop();
+    Timer tr(c->stream);
+    do_restrict(c);
+    const double t_re = tr.stop();
+    Timer tf(c->stream);
+    do_factor(c);
+    const double t_fa = tf.stop();
+    do_patera(c, U, P, opts, info, opts->warm_U ? U0_dev : nullptr, P0_dev);
     if (info) {
       info->t_assemble = t_as;
       info->t_restrict = t_re;

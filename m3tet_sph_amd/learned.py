@@ -61,10 +61,11 @@ LIB.m3_gnn_push_state.argtypes = [C.c_void_p, C.c_double, D, D, D]
 LIB.m3_gnn_extrapolate.argtypes = [C.c_void_p, C.c_double, C.c_int32, D, D, D]
 LIB.m3_gnn_add_sample_hist.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, D, D, D, D]
 LIB.m3_gnn_warm_start_hist.argtypes = [C.c_void_p, C.c_double, C.c_int32, D, D, D, D]
+LIB.m3_gnn_start_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
 for _f in ("m3_gnn_create", "m3_gnn_set_params", "m3_gnn_get_params", "m3_gnn_features", "m3_gnn_reserve",
            "m3_gnn_add_sample", "m3_gnn_fit_norm", "m3_gnn_train", "m3_gnn_eval", "m3_gnn_forward_backward",
            "m3_gnn_warm_start", "m3_gnn_sample_stats", "m3_gnn_clear_history", "m3_gnn_push_state",
-           "m3_gnn_extrapolate", "m3_gnn_add_sample_hist", "m3_gnn_warm_start_hist"):
+           "m3_gnn_extrapolate", "m3_gnn_add_sample_hist", "m3_gnn_warm_start_hist", "m3_gnn_start_device"):
     getattr(LIB, _f).restype = C.c_int
 
 
@@ -228,9 +229,13 @@ class WarmStartNet:
         a = [self._d(v) for v in (T, U, P)]
         self._check(LIB.m3_gnn_push_state(self._h, float(t), *[_p(v) for v in a]))
 
-    def extrapolate(self, t, order=2):
+    def extrapolate(self, t, order=2, device=False):
         """Base state (T, U, P) at time t: Lagrange extrapolation of degree
-        min(order, pushed - 1) through the most recent pushed states."""
+        min(order, pushed - 1) through the most recent pushed states; with
+        device=True it stays on the GPU as the start (start_device())."""
+        if device:
+            self._check(LIB.m3_gnn_extrapolate(self._h, float(t), int(order), None, None, None))
+            return None
         Tb, Ub, Pb = np.zeros(self.n), np.zeros(3 * self.mesh.nnod), np.zeros(self.n)
         self._check(LIB.m3_gnn_extrapolate(self._h, float(t), int(order), _p(Tb), _p(Ub), _p(Pb)))
         return Tb, Ub, Pb
@@ -239,14 +244,25 @@ class WarmStartNet:
         a = [self._d(v) for v in (T, Visc, U_next, P_next)]
         self._check(LIB.m3_gnn_add_sample_hist(self._h, int(idx), float(t), int(order), *[_p(v) for v in a]))
 
-    def warm_start_hist(self, t, order, T, Visc):
-        """Network start from the extrapolated base state at time t."""
+    def warm_start_hist(self, t, order, T, Visc, device=False):
+        """Network start from the extrapolated base state at time t; with
+        device=True it stays on the GPU (start_device()) and None is returned."""
         a = [self._d(v) for v in (T, Visc)]
+        if device:
+            self._check(LIB.m3_gnn_warm_start_hist(self._h, float(t), int(order), *[_p(v) for v in a], None, None))
+            return None
         U0 = np.zeros(3 * self.mesh.nnod)
         P0 = np.zeros(self.n)
         self._check(LIB.m3_gnn_warm_start_hist(self._h, float(t), int(order), *[_p(v) for v in a], _p(U0),
                                                _p(P0)))
         return U0, P0
+
+    def start_device(self):
+        """Device addresses (U0, P0) of the last warm start, for
+        StokesContext.flow3d(start_device=...)."""
+        u, p = C.c_void_p(), C.c_void_p()
+        self._check(LIB.m3_gnn_start_device(self._h, C.byref(u), C.byref(p)))
+        return u.value, p.value
 
     def save(self, path, meta=None):
         p, q = self.get_params()

@@ -117,6 +117,43 @@ def test_history_extrapolation_and_warm_start_match_oracle(c0, c0_inputs, net0):
     net0.clear_history()
 
 
+def test_device_start_equals_host_start(c0, c0_inputs, net0):
+    """The extrapolated and the network start handed to the solver on the
+    device (m3_gnn_start_device + m3_flow3d_patera_sph_dstart) give the same
+    solve as the same starts passed through host memory (to the run-to-run
+    spread that the assembly's FP64 atomics leave); a host address is
+    refused as a device start."""
+    from m3tet_sph_amd import api
+    from m3tet_sph_amd.api import M3Error
+    inp = c0_inputs
+    nV = c0.nVnod
+    ctx = api.StokesContext(c0)
+    ctx.set_bc(inp["ifix"], inp["vfix"])
+    args = (inp["Visc"], inp["Dens"], inp["phys"])
+    U, P, i0 = ctx.flow3d(*args, rtol_Pat=1e-8, rtol_U_final=1e-8)
+    net0.clear_history()
+    for j, tj in enumerate((0.0, 1e-3, 2.5e-3)):
+        net0.push_state(tj, inp["T"][:nV] * (1 + 0.01 * j), U * (1 + 0.01 * j), P * (1 - 0.01 * j))
+    _, _, p, norm = _random_problem(nV, 14)
+    p[OG.OFF_WD:] *= 0.01
+    net0.set_params(p, norm)
+    t, T, V = 3.5e-3, inp["T"][:nV] * 1.03, inp["Visc"]
+    opts = dict(rtol_U_final=1e-7, tol_Pat_abs=1e-7 * i0["norm_rP0"])
+    starts = ((lambda: net0.extrapolate(t, 2, device=True), lambda: net0.extrapolate(t, 2)[1:]),
+              (lambda: net0.warm_start_hist(t, 2, T, V, device=True), lambda: net0.warm_start_hist(t, 2, T, V)))
+    for dev, host in starts:
+        U0, P0 = host()
+        Uh, Ph, ih = ctx.flow3d(*args, U0=U0, P0=P0, **opts)
+        dev()
+        Ud, Pd, idv = ctx.flow3d(*args, start_device=net0.start_device(), **opts)
+        assert relerr(Uh, Ud) < 1e-9 and relerr(Ph, Pd) < 1e-9, (relerr(Uh, Ud), relerr(Ph, Pd))
+        assert abs(ih["nitZ"] + ih["nitU"] - idv["nitZ"] - idv["nitU"]) <= 1
+    with pytest.raises(M3Error):
+        ctx.flow3d(*args, start_device=(U0.ctypes.data, P0.ctypes.data), **opts)
+    net0.clear_history()
+    ctx.close()
+
+
 def test_training_reduces_loss(c0, c0_inputs, net0):
     """Targets that are a smooth function of the features are learned."""
     from m3tet_sph_amd import learned as L
