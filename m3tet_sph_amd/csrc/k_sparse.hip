@@ -763,6 +763,15 @@ double mean(m3_ctx *c, const double *a, int64_t n) {
   return c->h_scal[8] / (double)n;
 }
 
+// bit 32 of flag when x holds a NaN
+__global__ void k_nan_flag(int64_t n, const double *x, int *flag) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && x[i] != x[i]) atomicOr(flag, 32);
+}
+void nan_flag(hipStream_t s, const double *x, int64_t n, int *flag) {
+  hipLaunchKernelGGL(k_nan_flag, dim3(nblk(n)), dim3(256), 0, s, n, x, flag);
+}
+
 void axpy(hipStream_t s, double a, const double *x, double *y, int64_t n) {
   hipLaunchKernelGGL(k_axpy, dim3(nblk(n)), dim3(256), 0, s, n, a, x, y);
 }

@@ -121,6 +121,7 @@ void raise_flags(int e) {
   if (e & 4) throw M3Error(M3_E_NUMERIC, "Calculated a zero or negative viscosity");
   if (e & 8) throw M3Error(M3_E_MESH, "element block missing from the sparse pattern");
   if (e & 16) throw M3Error(M3_E_NUMERIC, "coarse matrix is not positive definite");
+  if (e & 32) throw M3Error(M3_E_NUMERIC, "Solution contains NaNs");
 }
 
 template <class F>
@@ -752,13 +753,12 @@ static void do_patera(m3_ctx *c, double *U, double *P, const m3_patera_opts *o, 
   // U = Uf + U0 (rotated frame), P
   m3::copy(s, Uf, Y, n3);
   m3::axpy(s, 1.0, c->U0.p, Y, n3);
+  c->errflag.zero(s);  // NaN test on the device (flow3d_patera_mg_sph_p.m:663)
+  m3::nan_flag(s, Y, n3, c->errflag.p);
+  m3::nan_flag(s, Pd, nV, c->errflag.p);
   HIPCHECK(hipMemcpyAsync(U, Y, sizeof(double) * n3, hipMemcpyDeviceToHost, s));
   HIPCHECK(hipMemcpyAsync(P, Pd, sizeof(double) * nV, hipMemcpyDeviceToHost, s));
-  HIPCHECK(hipStreamSynchronize(s));
-  for (int64_t i = 0; i < nV; ++i)
-    if (std::isnan(P[i])) throw M3Error(M3_E_NUMERIC, "Solution contains NaNs");
-  for (int64_t i = 0; i < n3; ++i)
-    if (std::isnan(U[i])) throw M3Error(M3_E_NUMERIC, "Solution contains NaNs");
+  raise_flags(check_err(c));  // synchronises the stream
   if (info) {
     info->nitP = itPat;
     info->nitU = nitU;

@@ -280,6 +280,7 @@ void launch_diag_inv(m3_ctx *c, int l);
 void launch_lumped_mm(m3_ctx *c);
 
 // sparse kernels
+void nan_flag(hipStream_t s, const double *x, int64_t n, int *flag);  // atomicOr(flag, 32) on a NaN
 void spmv_K(hipStream_t s, LevelDev &lv, const double *x, double *y);
 void spmv_G(hipStream_t s, Sell &G, const double *q, double *y, const double *mask);
 void spmv_GT(hipStream_t s, Sell &GT, const double *z, double *y);
