@@ -101,16 +101,16 @@ def main(argv=None):
             rP_cold = info["norm_rP0"]
         else:
             tol_abs = 0.8 * a.tol * rP_cold
-            starts = []
+            starts = []  # formed on the GPU and handed to the solver there
             if a.extrap >= 0:
-                starts.append(("extrap", lambda: hist.extrapolate(time_, order)[1:]))
+                starts.append(("extrap", lambda: hist.extrapolate(time_, order, device=True)))
             if net is not None:
-                starts.append(("net", lambda: net.warm_start_hist(time_, order, Tv, V)))
+                starts.append(("net", lambda: net.warm_start_hist(time_, order, Tv, V, device=True)))
             for name, start in starts:
                 t = time.perf_counter()
-                U0, P0 = start()
-                Un, Pn, i2 = ctx.flow3d(V, Dn, phys, P0=P0, U0=U0, precision=a.form, rtol_U_final=0.5 * a.tol,
-                                        tol_Pat_abs=tol_abs)
+                start()
+                Un, Pn, i2 = ctx.flow3d(V, Dn, phys, start_device=hist.start_device(), precision=a.form,
+                                        rtol_U_final=0.5 * a.tol, tol_Pat_abs=tol_abs)
                 rU, fU, rP = api.residual(ctx, Un, Pn)
                 row[name] = dict(s=time.perf_counter() - t, nitP=i2["nitP"], nitZ=i2["nitZ"], nitU=i2["nitU"],
                                  rel_res=max(rU / fU, rP / rP_cold))

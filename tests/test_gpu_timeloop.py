@@ -91,14 +91,18 @@ def test_short_time_loop_on_device_fields(c0, c0_inputs, ctx):
 
 
 def test_timeloop_driver_snapshots_and_restart(tmp_path):
-    """The time-loop driver on c0 with diffusion, snapshot / restart files in
-    the reference's MAT layout, and a restart from the last snapshot."""
+    """The time-loop driver on c0 with diffusion, a second solve of every
+    step from the extrapolated start (formed on the device), snapshot /
+    restart files in the reference's MAT layout, and a restart from the last
+    snapshot."""
     import json
     from m3tet_sph_amd import matio, timeloop
     out = str(tmp_path)
-    timeloop.main(["--config", "c0", "--steps", "3", "--kappa", "1e2", "--snapshots", "2", "--out", out])
+    timeloop.main(["--config", "c0", "--steps", "3", "--kappa", "1e2", "--snapshots", "2", "--extrap", "2",
+                   "--out", out])
     summ = json.load(open(os.path.join(out, "timeloop_c0.json")))["summary"]
     assert summ["max_rel_res"] <= 1e-6 and summ["thermal"]["nit"] > 0
+    assert summ["extrap"]["max_rel_res"] <= 1e-6
     files = sorted(os.listdir(out))
     assert "c0_Data_0001.mat" in files and "c0_Data_0002.mat" in files
     assert "c0_RestartFile1.mat" in files and "c0_RestartFile2.mat" in files
