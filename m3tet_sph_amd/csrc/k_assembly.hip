@@ -483,9 +483,19 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
   __shared__ int s_nod[10];
   __shared__ double s_m[30];
   if (lane < 10) s_nod[lane] = a.el2nod_c[c * 10 + lane];
-  for (int t = lane; t < 900; t += 64) s_Kc[t / 30][t % 30] = 0.0;
   __syncthreads();
   if (lane < 30) s_m[lane] = a.mask_c[3 * (int64_t)s_nod[lane / 3] + lane % 3];
+  // Register blocking: lane task u (< 90, two per lane) owns in the first
+  // product the row p = u / 3, component d = u % 3 of T (ten entries, one
+  // per coarse node b) and in the second the component c = u / 30, column
+  // q = u % 30 of Kc (ten entries, one per coarse node a), accumulated over
+  // the eight children in registers; Q is read from LDS by broadcast.
+  // Every entry sums in the same order as an entry-per-lane loop would.
+  double kc[2][10];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int aa = 0; aa < 10; ++aa) kc[r][aa] = 0.0;
   for (int k = 0; k < 8; ++k) {
     const int64_t e = 8 * c + k;
     for (int t = lane; t < 465; t += 64) {
@@ -498,22 +508,54 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
     for (int t = lane; t < 100; t += 64) s_Q[t / 10][t % 10] = Q[t];
     __syncthreads();
     // T[p][(b,d)] = sum_j Kf[p][(j,d)] Q[j][b]
-    for (int t = lane; t < 900; t += 64) {
-      const int p = t / 30, q = t % 30, b = q / 3, d = q % 3;
-      double acc = 0;
-      for (int j = 0; j < 10; ++j) acc += s_Kf[p][3 * j + d] * s_Q[j][b];
-      s_T[p][q] = acc;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int u = lane + 64 * r;
+      if (u < 90) {
+        const int p = u / 3, d = u % 3;
+        double kf[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) kf[j] = s_Kf[p][3 * j + d];
+#pragma unroll
+        for (int bb = 0; bb < 10; ++bb) {
+          double acc = 0;
+#pragma unroll
+          for (int j = 0; j < 10; ++j) acc += kf[j] * s_Q[j][bb];
+          s_T[p][3 * bb + d] = acc;
+        }
+      }
     }
     __syncthreads();
     // Kc[(a,c)][q] += sum_i Q[i][a] T[(i,c)][q]
-    for (int t = lane; t < 900; t += 64) {
-      const int p = t / 30, q = t % 30, aa = p / 3, cc = p % 3;
-      double acc = 0;
-      for (int i = 0; i < 10; ++i) acc += s_Q[i][aa] * s_T[3 * i + cc][q];
-      s_Kc[p][q] += acc;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int u = lane + 64 * r;
+      if (u < 90) {
+        const int cc = u / 30, q = u % 30;
+        double tv[10];
+#pragma unroll
+        for (int i = 0; i < 10; ++i) tv[i] = s_T[3 * i + cc][q];
+#pragma unroll
+        for (int aa = 0; aa < 10; ++aa) {
+          double acc = 0;
+#pragma unroll
+          for (int i = 0; i < 10; ++i) acc += s_Q[i][aa] * tv[i];
+          kc[r][aa] += acc;
+        }
+      }
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int u = lane + 64 * r;
+    if (u < 90) {
+      const int cc = u / 30, q = u % 30;
+#pragma unroll
+      for (int aa = 0; aa < 10; ++aa) s_Kc[3 * aa + cc][q] = kc[r][aa];
+    }
+  }
+  __syncthreads();
   for (int t = lane; t < 900; t += 64) {
     const int p = t / 30, q = t % 30;
     s_Kc[p][q] *= s_m[p] * s_m[q];
