@@ -46,20 +46,23 @@ inline int nblk(int64_t n) { return (int)((n + 255) / 256); }
 // ---------------------------------------------------------------- features
 // s_u, s_p: RMS vertex speed and pressure of the base state; s_T: RMS of the
 // temperature's departure from the base state's temperature
-__global__ void k_scales(int64_t n, const double *U, const double *P, const double *T, const double *Tb,
-                         double *sc) {
+// (one workgroup of SCB threads: a fixed summation order, so the scales do
+// not depend on the launch)
+constexpr int SCB = 1024;
+__global__ __launch_bounds__(SCB) void k_scales(int64_t n, const double *U, const double *P, const double *T,
+                                                const double *Tb, double *sc) {
   double su = 0, sp = 0, st = 0;
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
     su += U[3 * i] * U[3 * i] + U[3 * i + 1] * U[3 * i + 1] + U[3 * i + 2] * U[3 * i + 2];
     sp += P[i] * P[i];
     st += (T[i] - Tb[i]) * (T[i] - Tb[i]);
   }
-  __shared__ double a[256], b[256], c[256];
+  __shared__ double a[SCB], b[SCB], c[SCB];
   a[threadIdx.x] = su;
   b[threadIdx.x] = sp;
   c[threadIdx.x] = st;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = SCB / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       a[threadIdx.x] += a[threadIdx.x + o];
       b[threadIdx.x] += b[threadIdx.x + o];
@@ -541,7 +544,7 @@ void upload_state(m3_gnn *g, const double *T, const double *Tp, const double *Vi
   HIPCHECK(hipMemcpyAsync(g->visc.p, Visc, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(g->Up.p, Up, sizeof(double) * 3 * g->nnod, hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(g->Pp.p, Pp, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
+  hipLaunchKernelGGL(k_scales, dim3(1), dim3(SCB), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
 }
 
 void features_to(m3_gnn *g, float *X) {
@@ -591,7 +594,7 @@ void make_base(m3_gnn *g, double t, int order, bool scales = true) {
   hipLaunchKernelGGL(k_lagrange, dim3(nblk(3 * nn)), dim3(256), 0, s, 3 * nn, lu, g->Up.p);
   hipLaunchKernelGGL(k_lagrange, dim3(nblk(n)), dim3(256), 0, s, n, lp, g->Pp.p);
   if (scales)  // needs the new temperature in T
-    hipLaunchKernelGGL(k_scales, dim3(1), dim3(256), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
+    hipLaunchKernelGGL(k_scales, dim3(1), dim3(SCB), 0, s, n, g->Up.p, g->Pp.p, g->T.p, g->Tp.p, g->sc.p);
 }
 
 // U0 = Up + network change (vertices; edge nodes take the mean of their two
