@@ -17,9 +17,9 @@
 // 233-243, lifts the prescribed velocities into bc_rhs (:280-283), removes
 // the fixed dofs (:297-298) and scatter-adds the G, GT and M blocks into the
 // sliced block-sparse operators.  The rotated, masked element matrix Kel is
-// kept; the finest K is summed from it row by row (k_gatherK, no global
-// atomics) and the element-local Galerkin product of the next multigrid level
-// reads it too.
+// kept (node blocks a >= b, KEL layout); the finest K is summed from it row
+// by row (k_gatherK, no global atomics) and the element-local Galerkin
+// product of the next multigrid level reads it too.
 #include "m3_internal.h"
 
 #include <cmath>
@@ -35,7 +35,7 @@ __constant__ double c_N4[NIP][4];
 __constant__ double c_N20[NIP][20];
 __constant__ double c_dN20[NIP][20][3];
 __constant__ double c_dN4[4][3];
-__constant__ uint16_t c_pq[465];  // packed-lower index -> (p | q<<8), column-major
+__constant__ uint16_t c_bq[KEL];  // stored element-matrix index -> (p | q<<8), block layout (KEL)
 
 // ---- host restatement of the shape-function tables --------------------------
 
@@ -415,10 +415,10 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
     s_K[p][q] *= s_msk[p] * s_msk[q];
   }
   __syncthreads();
-  if (a.Kel) {
-    for (int t = lane; t < 465; t += 64) {
-      const int pq = c_pq[t];
-      a.Kel[e * 465 + t] = s_K[pq & 255][pq >> 8];
+  if (a.Kel) {  // lower-triangle values, so that the stored diagonal blocks are symmetric
+    for (int t = lane; t < KEL; t += 64) {
+      const int pq = c_bq[t], p = pq & 255, q = pq >> 8;
+      a.Kel[e * KEL + t] = p >= q ? s_K[p][q] : s_K[q][p];
     }
   }
   // ---------------- scatter K blocks (unless the row-owner gather k_gatherK sums them)
@@ -463,12 +463,12 @@ __global__ __launch_bounds__(64) void k_assemble(AsmArgs a) {
 // interpolation rows of Ic2f are the parent's shape functions.
 struct RapArgs {
   int64_t nel_c;
-  const double *Kf;        // fine element matrices (packed 465)
+  const double *Kf;        // fine element matrices (KEL layout)
   const int32_t *ctype;    // fine element -> Q table
   const double *Qtab;      // ntypes * 100, Q[i][a] = weight of coarse local a at fine local i
   const int32_t *el2nod_c; // coarse connectivity
   const double *mask_c;
-  double *Kc;              // coarse element matrices (packed 465)
+  double *Kc;              // coarse element matrices (KEL layout)
   SellView K;              // coarse operator
   int *err;
 };
@@ -498,9 +498,9 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
     for (int aa = 0; aa < 10; ++aa) kc[r][aa] = 0.0;
   for (int k = 0; k < 8; ++k) {
     const int64_t e = 8 * c + k;
-    for (int t = lane; t < 465; t += 64) {
-      const int pq = c_pq[t];
-      const double v = a.Kf[e * 465 + t];
+    for (int t = lane; t < KEL; t += 64) {  // symmetric pairs of a diagonal block carry equal values
+      const int pq = c_bq[t];
+      const double v = a.Kf[e * KEL + t];
       s_Kf[pq & 255][pq >> 8] = v;
       s_Kf[pq >> 8][pq & 255] = v;
     }
@@ -562,10 +562,10 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
   }
   __syncthreads();
   if (a.Kc) {
-    for (int t = lane; t < 465; t += 64) {
-      const int pq = c_pq[t];
+    for (int t = lane; t < KEL; t += 64) {
+      const int pq = c_bq[t];
       // symmetrise the rounding of the two triangles
-      a.Kc[c * 465 + t] = 0.5 * (s_Kc[pq & 255][pq >> 8] + s_Kc[pq >> 8][pq & 255]);
+      a.Kc[c * KEL + t] = 0.5 * (s_Kc[pq & 255][pq >> 8] + s_Kc[pq >> 8][pq & 255]);
     }
   }
   for (int t = lane; t < 100; t += 64) {
@@ -584,8 +584,9 @@ __global__ __launch_bounds__(64) void k_rap(RapArgs a) {
 // atomics: one wavefront per node row i accumulates its blocks in LDS.  Lane
 // work item = (incident element e with i at local node a, local column node
 // b); the 3x3 block (a,b) of Kel lands in stored block k of row i, where the
-// slot of (a,b) comes from the per-element slot map.  Kel is the packed lower
-// (column-major) triangle of the symmetric 30x30 rotated, masked matrix.
+// slot of (a,b) comes from the per-element slot map.  Kel holds the node
+// blocks a >= b of the symmetric 30x30 rotated, masked matrix (KEL layout),
+// so a work item reads one contiguous 72-byte block.
 // Every stored block of the row is written, so K needs no clearing first.
 constexpr int GK_ROWS = 4;      // rows (wavefronts) per block
 constexpr int GK_MAXLEN = 128;  // stored blocks per row (checked at m3_set_mesh)
@@ -610,13 +611,17 @@ __global__ __launch_bounds__(64 * GK_ROWS) void k_gatherK(SellView K, int64_t nr
       const int64_t e = ea / 10;
       const int na = ea - 10 * (int)e;
       const int k = (int)((slots[e * ASM_SLOTS + 10 * na + b] - base) >> 6);
-      const double *ke = Kel + e * 465;
+      // block (na, b) of the symmetric element matrix: stored block (hi, lo), transposed when na < b
+      const bool lower = na >= b;
+      const int hi = lower ? na : b, lo = lower ? b : na;
+      const double *kb = Kel + e * KEL + 9 * (hi * (hi + 1) / 2 + lo);
+      double v[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) v[m] = kb[m];
+#pragma unroll
       for (int cc = 0; cc < 3; ++cc)
-        for (int dd = 0; dd < 3; ++dd) {
-          const int p = 3 * na + cc, q = 3 * b + dd;
-          const int lo = p < q ? p : q, hi = p < q ? q : p;
-          atomicAdd(&A[9 * k + 3 * cc + dd], ke[lo * 30 - lo * (lo - 1) / 2 + (hi - lo)]);
-        }
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) atomicAdd(&A[9 * k + 3 * cc + dd], lower ? v[3 * cc + dd] : v[3 * dd + cc]);
     }
   }
   __syncthreads();
@@ -694,10 +699,13 @@ void upload_sf_tables() {
     sf20(r, s, t, N20[ip], dN20[ip]);
   }
   const double dN4[4][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, -1, -1}};
-  uint16_t pq[465];
+  uint16_t bq[KEL];
   int idx = 0;
-  for (int q = 0; q < 30; ++q)
-    for (int p = q; p < 30; ++p) pq[idx++] = (uint16_t)(p | (q << 8));
+  for (int na = 0; na < 10; ++na)
+    for (int nb = 0; nb <= na; ++nb)
+      for (int cc = 0; cc < 3; ++cc)
+        for (int dd = 0; dd < 3; ++dd) bq[idx++] = (uint16_t)((3 * na + cc) | ((3 * nb + dd) << 8));
+  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_bq), bq, sizeof(bq)));
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_ipw), ipw, sizeof(ipw)));
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N10), N10, sizeof(N10)));
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN10), dN10, sizeof(dN10)));
@@ -705,7 +713,6 @@ void upload_sf_tables() {
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_N20), N20, sizeof(N20)));
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN20), dN20, sizeof(dN20)));
   HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_dN4), dN4, sizeof(dN4)));
-  HIPCHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_pq), pq, sizeof(pq)));
 }
 
 void launch_assemble(m3_ctx *c, double *dbgK, double *dbgG, double *dbgM, double *dbgF,

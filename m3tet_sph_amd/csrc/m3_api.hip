@@ -234,7 +234,7 @@ int m3_set_mesh(m3_ctx *c, const m3_mesh_desc *m) {
       std::vector<int32_t> cols;
       adjacency(lv.h_el2nod, lv.nel, lv.nnod, 10, ALL10, 10, ALL10, ptr, cols);
       lv.K.build(lv.nnod, lv.nnod, ptr, cols, 9);
-      if (l < nmg - 1) lv.Kel.alloc((size_t)lv.nel * 465);
+      if (l < nmg - 1) lv.Kel.alloc((size_t)lv.nel * KEL);
       mark("level connectivity + K pattern");
     }
     LevelDev &f = c->L[0];

@@ -150,6 +150,12 @@ struct CholTiles {
 // [100+4a+v], GT 4x10 at [140+4a+v], M 4x4 at [180+4v+w]
 constexpr int ASM_SLOTS = 196;
 
+// stored element matrix: the 55 node blocks (a >= b) of the symmetric 30x30
+// matrix, block a(a+1)/2 + b, each 3x3 row-major (entry [c][d] = K[3a+c][3b+d]);
+// the diagonal blocks are stored whole (and symmetric), so every block is one
+// contiguous 72-byte read
+constexpr int KEL = 55 * 9;
+
 struct LevelDev {
   int64_t nel = 0, nnod = 0, nV = 0;
   std::vector<int32_t> h_el2nod;   // 0-based nel*10
@@ -158,7 +164,7 @@ struct LevelDev {
   DBuf<double> mask;               // 3*nnod, 1 free / 0 fixed
   DBuf<double> dinv;               // 3*nnod inverse diagonal (1 at fixed dofs)
   DBuf<float> dinv32;
-  DBuf<double> Kel;                // nel*465 rotated + masked element matrices
+  DBuf<double> Kel;                // nel*KEL rotated + masked element matrices
   // transfer to this level from level+1 (fine rows) and its transpose
   DBuf<int64_t> P_ptr, R_ptr;
   DBuf<int32_t> P_col, R_col;
